@@ -1,0 +1,154 @@
+"""Benchmark of the particle-history path on one MI355X.
+
+Workload: the Al2O3-like target (Egap 8.8 eV, 4 VB + Al L + O K oscillator
+sets, 3.95 g/cm^3, 100 A layer) hit by Au (Z 79) at 11 MeV/u, FP64, tallies on
+1000 time nodes (0-1000 fs, dt 1 fs) x 100 log radial bins (1-1000 A).  Tables
+are built once before timing (resident in HBM).  One step = trk_run_mc over
+--iters-per-step Monte Carlo iterations (whole ion histories with all their
+cascades), with distinct iteration numbers so every step draws fresh random
+streams.  The metric is collision events per second (every SHI ionisation,
+electron/hole elastic or inelastic collision, Auger or radiative decay and
+photoabsorption counts once).
+
+Also reported:
+  roofline      the transport kernel's algorithmic bytes (per-event state
+                read/write, spawned particles, table gathers, per-time-node
+                snapshots of live particles) per launch over its mean launch
+                time, HIP events on the library's stream, against 8 TB/s
+  cpu_baseline  the oracle (sequential C restatement, one core) replaying one
+                iteration of the same workload on the same tables
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "trekis-3_amd"))
+
+HBM_PEAK_GBS = 8000.0
+
+# bytes of the algorithmic model (per event, per spawn, per lookup, per snapshot)
+STATE_E, STATE_H = 72, 96          # 9 f64 / 11 f64 + 2 i32
+SPAWN_PAIR, SPAWN_PHOT = STATE_E + STATE_H, STATE_E
+GATHER = 16                        # two bracketing table entries
+SNAP_E, SNAP_H = 48, 72
+
+
+def transport_bytes(ev, nsh, snaps_e, snaps_h, snaps_p, phonon_rows):
+    """Algorithmic bytes of the transport kernel's events (the ion kernel's
+    SHI events excluded)."""
+    e_inel, e_el, h_inel, h_el, aug, rad, phot = (int(ev[i]) for i in range(1, 8))
+    b = (e_inel + e_el) * 2 * STATE_E + (h_inel + h_el + aug + rad) * 2 * STATE_H + phot * 2 * STATE_E
+    b += (e_inel + h_inel + aug + phot) * SPAWN_PAIR + rad * SPAWN_PHOT
+    g = e_inel * (2 + nsh + 2) + e_el * (2 + (2 if phonon_rows else 0)) + h_inel * (2 + nsh + 2) \
+        + h_el * (2 + (2 if phonon_rows else 0)) + (aug + rad) * nsh + phot * (1 + nsh)
+    b += g * GATHER
+    b += snaps_e * SNAP_E + snaps_h * SNAP_H + snaps_p * SNAP_E
+    return b
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--iters-per-step", type=int, default=64)
+    ap.add_argument("--cpu-iters", type=int, default=1, help="oracle iterations for the CPU baseline (0 = skip)")
+    ap.add_argument("--config", default="config2")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); the multi-GPU split of "
+              f"iterations is not built in this tree", file=sys.stderr)
+        sys.exit(1)
+
+    import trekis_gpu as tg
+    from trekis_gpu import abi
+    lib = tg.load_library()
+    mat, ion, opt, run = tg.config_material(args.config)
+    mat.prepare(lib)
+    ctx = tg.Context(mat, ion, opt)
+    t0 = time.perf_counter()
+    ctx.build_tables()
+    t_tables = time.perf_counter() - t0
+    print(f"[bench] tables built in {t_tables:.2f} s", file=sys.stderr, flush=True)
+    R, V = tg.radial_grid(run["NR"], run["Rmin"], run["Rmax"], mat.layer)
+    B = args.iters_per_step
+    seed = run["seed"]
+    it = 0
+    for w in range(args.warmup):
+        ctx.run_mc(B, seed, R, V, first_iter=it)
+        it += B
+        print(f"[bench] warmup step {w} done", file=sys.stderr, flush=True)
+
+    ctx.set_profiling(True)
+    ev = np.zeros(8, dtype=np.int64)
+    snaps = [0.0, 0.0, 0.0]
+    t_total = 0.0
+    for k in range(args.steps):
+        t = ctx.new_tallies(len(R))
+        ts = time.perf_counter()
+        ctx.run_mc(B, seed, R, V, tallies=t, first_iter=it)
+        t_total += time.perf_counter() - ts
+        it += B
+        ev += t.n_events
+        snaps[0] += float(t["Out_tot_Ne"].sum())
+        snaps[1] += float((t["Out_nh"] / V[None, :, None, None]).sum())
+        snaps[2] += float(t["Out_tot_Nphot"].sum())
+        print(f"[bench] step {k}: {int(t.n_events.sum())} events", file=sys.stderr, flush=True)
+    ms_tr, n_tr = ctx.kernel_time(abi.K_TRANSPORT)
+    ms_shi, n_shi = ctx.kernel_time(abi.K_SHI)
+    ctx.set_profiling(False)
+
+    total_ev = int(ev.sum())
+    value = total_ev / t_total
+    nsh = sum(mat.n_shells)
+    b_tr = transport_bytes(ev, nsh, *[int(round(s)) for s in snaps], opt.kind_of_EMFP == 1)
+    per_launch = b_tr / max(n_tr, 1)
+    mean_s = (ms_tr / max(n_tr, 1)) * 1e-3
+    achieved = per_launch / mean_s / 1e9 if mean_s > 0 else 0.0
+
+    cpu = None
+    if args.cpu_iters > 0:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        from oracle import Oracle
+        orc = Oracle(mat, ion, opt)
+        orc.inject_from(ctx)
+        ts = time.perf_counter()
+        c = orc.run_mc(args.cpu_iters, seed, R, V, tg.TallySet(orc.Nt, len(R), mat))
+        dt = time.perf_counter() - ts
+        cpu = {"value": float(c.n_events.sum()) / dt, "unit": "events/s", "cores": 1, "kind": "port",
+               "sample": f"{args.cpu_iters} iteration(s) of the same workload on the GPU-built tables, "
+                         f"{int(c.n_events.sum())} events in {dt:.1f} s, sequential C oracle"}
+
+    line = {
+        "metric": "collision events/s on 1 GPU (FP64 path)",
+        "value": value,
+        "unit": "events/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1e3 * t_total / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": f"{mat.name} target, Au 11 MeV/u ion, 0-1000 fs x 100 radial bins, "
+                               f"{B} iterations per step", "iters_per_step": B, "events": total_ev,
+                   "events_by_kind": dict(zip(abi.EVENT_NAMES, ev.tolist())), "table_build_s": t_tables,
+                   "shi_kernel_ms": ms_shi, "shi_launches": n_shi},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": "k_transport", "launches": n_tr, "mean_launch_ms": ms_tr / max(n_tr, 1),
+                     "alg_bytes_per_launch": per_launch},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

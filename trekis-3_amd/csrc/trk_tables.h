@@ -1,0 +1,70 @@
+// trk_tables.h -- device table layout and the table-build interface.
+#pragma once
+
+#include <stdint.h>
+#include <vector>
+#include <string>
+#include "trk_dev_common.h"
+
+namespace trk {
+
+// Material view handed to kernels by value (pointers are device memory).
+struct MatDev {
+    int nsh, vb, n_atoms, n_dos;
+    double Egap, Mtarget, Zt, sumPers;
+    double El_eff_mass, v_f, temp, N_VB_el, At_Dens, cut_off;
+    double hole_mass, Layer;
+    int kind_of_DR, use_k, n_ph, kind_of_EMFP, plasmon_Emax;
+    const int *osc_off;          // [nsh+1]
+    const double *oA, *oE0, *oG; // oscillators
+    const double *phA, *phE0, *phG;
+    const double *k, *Effm;      // DOS k / Eff_m (or the inverted pair for metals)
+    int nk;
+    const double *dosE, *dos, *dosI, *dosM;
+    // per shell
+    const double *Ip, *Nel, *Auger, *Radiat;
+    const int *atom_of, *shl_off;
+    const int *Zat;
+    const double *Mass, *Pers;
+    double ion_mass, ion_E, ion_Zeff0;
+    int ion_Z, ion_kind_zeff;
+    double ion_fixed_zeff;
+};
+
+// One table kind on the device.  Per-shell arrays are shell-major [nsh][n].
+struct TableDev {
+    int n = 0;
+    int nsh = 0;
+    double *E = nullptr;
+    double *L = nullptr;
+    double *dEdx = nullptr;
+    // cumulative rows: row (shell s, energy i) at offset row_off[s*n+i]
+    int64_t *row_off = nullptr;   // [nsh*n+1], device
+    double *hw = nullptr;
+    double *lam = nullptr;
+    std::vector<int64_t> row_off_h;   // host copy of offsets
+    std::vector<int> has_rows;        // per shell
+    // SHI_DIFF: per-shell rows of own length (E, L)
+    std::vector<int> dn;
+    std::vector<int64_t> doff;
+    double *dE = nullptr;
+    double *dL = nullptr;
+};
+
+struct TableSet {
+    TableDev tab[7];
+    double *el_imfp = nullptr;     // fused El_IMFP(2,:) lambda column
+    double *hole_imfp = nullptr;
+    double *phot_imfp = nullptr;
+    double *shi_path = nullptr;
+    int64_t launches = 0;
+    double build_seconds = 0.0;
+    int64_t dcs_evals = 0;         // number of q-integrals evaluated
+};
+
+struct HostMat;  // defined in trk_ctx.h
+
+int build_tables(const HostMat &hm, const MatDev &md, TableSet &ts, hipStream_t st, std::string &err);
+void free_tables(TableSet &ts);
+
+}  // namespace trk

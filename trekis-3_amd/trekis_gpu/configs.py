@@ -1,0 +1,304 @@
+"""Material, ion and run descriptors for the benchmark configurations.
+
+Input reading stays on the caller's side in the reference (Read_input_file,
+Reading_files_and_parameters.f90:162-634); this module plays that role for
+the Python host: it builds the flat trk_material from a readable description,
+normalises CDF oscillator strengths with the f-sum rule the way
+get_single_pole does (Cross_sections.f90:554-688), derives the VB density-of-
+states arrays (reading_material_DOS, RF:2374-2450) through the library, and
+builds the radial grid of Radius_for_distributions (Sorting_output_data.f90
+:1363-1435) for a caller-chosen set of radii.
+
+All choices the configurations leave open are fixed here and listed in the
+design notes (cut-off, time step, oscillator widths, DOS shapes, decay table).
+"""
+import ctypes
+import math
+from dataclasses import dataclass, field
+from typing import List, Tuple
+
+import numpy as np
+
+from . import abi
+
+# Universal_Constants.f90:28-60 (as folded by the library)
+G_ME = 9.1093821545e-31
+G_MP = 1836.1526724780 * G_ME
+G_H = 1.05457162853e-34
+G_E = 1.602176487e-19
+G_PI = 3.1415926535897932384626433832795
+
+
+@dataclass
+class Shell:
+    Ip: float
+    Nel: float
+    osc: List[Tuple[float, float, float]]       # (E0, A or weight, Gamma)
+    auger: float = 1.0e23                       # [fs]
+    radiat: float = 1.0e23                      # [fs]
+    shl_num: int = 1
+    normalize: bool = True                      # set A by the f-sum rule to Nel
+
+
+@dataclass
+class Atom:
+    Z: int
+    mass: float                                 # [proton masses]
+    pers: float
+    shells: List[Shell]
+
+
+@dataclass
+class MaterialDesc:
+    name: str
+    atoms: List[Atom]
+    dens: float                                 # [g/cm^3]
+    layer: float                                # [A]
+    vsound: float = 5000.0
+    E_F: float = 10.0
+    el_eff_mass: float = 1.0
+    hole_mass: float = 1.0
+    cut_off: float = 10.0
+    temp: float = 0.0
+    phonon: List[Tuple[float, float, float]] = field(default_factory=list)
+    phonon_single_pole: bool = False
+    dos_raw_E: np.ndarray = None
+    dos_raw: np.ndarray = None
+
+    def __post_init__(self):
+        self.n_atoms = len(self.atoms)
+        self.n_shells = [len(a.shells) for a in self.atoms]
+        sm = sum(a.mass * a.pers for a in self.atoms)
+        sp = sum(a.pers for a in self.atoms)
+        # At_Dens, RF:1335
+        self.At_Dens = 1.0e-3 * self.dens / (G_MP * sm / sp)
+        self.sum_pers = sp
+        self.v_f = math.sqrt(2.0 * self.E_F / G_ME)
+        vb = self.atoms[0].shells[-1]
+        self.N_VB_el = vb.Nel
+        self._lib = None
+        self.dos = None
+
+    # -------------------------------------------------------------- prep
+    def prepare(self, lib):
+        """Normalise oscillators and derive the DOS arrays (needs the library)."""
+        self._lib = lib
+        self._normalize(lib)
+        self._prepare_dos(lib)
+        return self
+
+    def _normalize(self, lib):
+        n_at_mol = self.sum_pers
+        for ia, atom in enumerate(self.atoms):
+            for js, sh in enumerate(atom.shells):
+                if not sh.normalize:
+                    continue
+                A = np.array([o[1] for o in sh.osc], dtype=np.float64)
+                E0 = np.array([o[0] for o in sh.osc], dtype=np.float64)
+                G = np.array([o[2] for o in sh.osc], dtype=np.float64)
+                is_vb = (ia == 0 and js == len(atom.shells) - 1)
+                if is_vb:
+                    target = sh.Nel / n_at_mol                       # CS:580
+                    Omega = lib.trk_w_plasma(1e6 * self.At_Dens, 0.0)  # CS:590
+                else:
+                    target = sh.Nel                                  # CS:600
+                    contrib = atom.pers / n_at_mol
+                    Omega = lib.trk_w_plasma(1e6 * self.At_Dens * contrib, 0.0)  # CS:613
+                ks = ctypes.c_double()
+                fs = ctypes.c_double()
+                rc = lib.trk_sumrules(A.ctypes.data_as(abi.DP), E0.ctypes.data_as(abi.DP), G.ctypes.data_as(abi.DP),
+                                      len(A), sh.Ip, Omega, ctypes.byref(ks), ctypes.byref(fs))
+                if rc:
+                    raise ValueError("sumrules failed")
+                A = A * (target / ks.value)
+                sh.osc = [(float(e), float(a), float(g)) for e, a, g in zip(E0, A, G)]
+                sh.normalize = False
+        if self.phonon_single_pole:
+            # get_single_pole phonon branch, CS:656-677
+            qd = (6.0 * G_PI * G_PI * (self.At_Dens * 1e6)) ** 0.33333333
+            Ed = G_H * self.vsound * qd / G_E
+            Ee = Ed * (G_PI / 6.0) ** (1.0 / 3.0)
+            E0 = 2.0 * Ee
+            Gm = E0 * 0.5
+            mean_mass = sum(a.pers * a.mass for a in self.atoms) * G_MP / n_at_mol
+            Omega = lib.trk_w_plasma(1e6 * self.At_Dens / n_at_mol, mean_mass)
+            A = np.array([1.0]); E = np.array([E0]); Gv = np.array([Gm])
+            ks = ctypes.c_double(); fs = ctypes.c_double()
+            lib.trk_sumrules(A.ctypes.data_as(abi.DP), E.ctypes.data_as(abi.DP), Gv.ctypes.data_as(abi.DP), 1, 1.0e-8,
+                             Omega, ctypes.byref(ks), ctypes.byref(fs))
+            self.phonon = [(E0, n_at_mol / ks.value, Gm)]
+            self.phonon_single_pole = False
+
+    def _prepare_dos(self, lib):
+        rE = np.ascontiguousarray(self.dos_raw_E, dtype=np.float64)
+        rD = np.ascontiguousarray(self.dos_raw, dtype=np.float64)
+        M = ctypes.c_int32()
+        if lib.trk_dos_size(rE.ctypes.data_as(abi.DP), len(rE), ctypes.byref(M)):
+            raise ValueError("bad DOS grid")
+        M = M.value
+        arrs = {k: np.zeros(M) for k in ["E", "DOS", "int_DOS", "k", "Eff_m", "k_inv", "Eff_m_inv"]}
+        rc = lib.trk_prepare_dos(rE.ctypes.data_as(abi.DP), rD.ctypes.data_as(abi.DP), len(rE), self.N_VB_el,
+                                 self.At_Dens, self.sum_pers,
+                                 *[arrs[k].ctypes.data_as(abi.DP) for k in ["E", "DOS", "int_DOS", "k", "Eff_m",
+                                                                             "k_inv", "Eff_m_inv"]], M)
+        if rc:
+            raise ValueError("trk_prepare_dos failed")
+        self.dos = arrs
+        self.n_dos = M
+
+    # -------------------------------------------------------------- struct
+    def to_struct(self):
+        if self.dos is None:
+            raise RuntimeError("MaterialDesc.prepare(lib) must run first")
+        m = abi.Material()
+        m.n_atoms = self.n_atoms
+        s = 0
+        o = 0
+        for ia, a in enumerate(self.atoms):
+            m.Zat[ia] = a.Z
+            m.Mass[ia] = a.mass
+            m.Pers[ia] = a.pers
+            m.n_shells[ia] = len(a.shells)
+            for js, sh in enumerate(a.shells):
+                m.Ip[s] = max(sh.Ip, 0.1)       # RF:1554
+                m.Nel[s] = sh.Nel
+                m.Auger[s] = sh.auger
+                m.Radiat[s] = sh.radiat
+                m.Shl_num[s] = 63 if (ia == 0 and js == len(a.shells) - 1) else sh.shl_num
+                m.n_osc[s] = len(sh.osc)
+                for (E0, A, G) in sh.osc:
+                    m.osc_E0[o], m.osc_A[o], m.osc_Gamma[o] = E0, A, G
+                    o += 1
+                s += 1
+        m.n_phonon = len(self.phonon)
+        for i, (E0, A, G) in enumerate(self.phonon):
+            m.ph_E0[i], m.ph_A[i], m.ph_Gamma[i] = E0, A, G
+        m.Dens = self.dens
+        m.At_Dens = self.At_Dens
+        m.Vsound = self.vsound
+        m.Layer = self.layer
+        m.N_VB_el = self.N_VB_el
+        m.El_eff_mass = self.el_eff_mass
+        m.hole_mass = self.hole_mass
+        m.E_F = self.E_F
+        m.v_f = self.v_f
+        m.cut_off = self.cut_off
+        m.temp = self.temp
+        m.work_function = 0.0
+        m.n_dos = self.n_dos
+        for k, f in [("E", "dos_E"), ("DOS", "dos_DOS"), ("int_DOS", "dos_int_DOS"), ("k", "dos_k"),
+                     ("Eff_m", "dos_Eff_m"), ("k_inv", "dos_k_inv"), ("Eff_m_inv", "dos_Eff_m_inv")]:
+            setattr(m, f, self.dos[k].ctypes.data_as(abi.DP))
+        self._struct = m
+        return m
+
+
+def radial_grid(NR, Rmin, Rmax, layer):
+    """Log-spaced cylinder radii Out_R [A] and inverse volumes Out_V [1/A^3]
+    (Out_V(j) = 1/((R_j^2 - R_{j-1}^2) pi L), Sorting_output_data.f90:1429)."""
+    R = np.array([10.0 ** (math.log10(Rmin) + (math.log10(Rmax) - math.log10(Rmin)) * j / (NR - 1)) for j in range(NR)])
+    V = np.zeros(NR)
+    R0 = 0.0
+    for j in range(NR):
+        V[j] = 1.0 / ((R[j] * R[j] - R0 * R0) * G_PI * layer)
+        R0 = R[j]
+    return R, V
+
+
+# -------------------------------------------------------------- DOS shapes
+def dos_insulator(width=22.0, step=0.05):
+    """Synthetic two-band oxide VB: an s-like band (Gaussian at -18 eV) under a
+    p-like band (half ellipse over the top 9 eV).  Energies relative to the VB top."""
+    E = np.arange(-width, 0.0 + step / 2, step)
+    E[-1] = 0.0
+    p = np.where(E > -9.0, np.sqrt(np.clip(1.0 - ((E + 4.5) / 4.5) ** 2, 0.0, None)), 0.0)
+    s = 0.35 * np.exp(-0.5 * ((E + 18.0) / 1.2) ** 2)
+    return E, p + s
+
+
+def dos_free_electron(E_F=11.17, step=0.05):
+    """Free-electron VB, DOS ~ sqrt(E - E_bottom), top at E_F."""
+    E = np.arange(-E_F, 0.0 + step / 2, step)
+    E[-1] = 0.0
+    return E, np.sqrt(np.clip(E + E_F, 0.0, None))
+
+
+# -------------------------------------------------------------- configurations
+def _al2o3_like(cut_off=10.0, layer=100.0, decay_seed=0x5EED0002, photons=True):
+    """Al2O3-like target of the 'insulator with hole kinetics' configuration."""
+    rng = np.random.default_rng(decay_seed)
+    br_L, br_K = rng.uniform(0.0, 1.0, size=2)
+    tau_L, tau_K = 20.6, 3.98          # total decay times [fs] (INPUT_CDF/Al2O3.cdf values)
+    def split(tau, br):
+        # total rate 1/tau split into radiative fraction br and Auger 1-br
+        return tau / (1.0 - br), (tau / br if photons else 1.0e23)
+    aL, rL = split(tau_L, br_L)
+    aK, rK = split(tau_K, br_K)
+    al = Atom(13, 26.98, 2.0, [
+        Shell(80.0, 8.0, [(90.0, 1.0, 40.0), (130.0, 0.6, 60.0)], auger=aL, radiat=rL, shl_num=2),
+        Shell(8.8, 24.0, [(14.0, 0.15, 3.0), (24.0, 0.35, 8.0), (33.0, 0.35, 12.0), (55.0, 0.15, 20.0)]),
+    ])
+    o = Atom(8, 16.00, 3.0, [
+        Shell(538.0, 2.0, [(560.0, 1.0, 120.0), (640.0, 0.5, 200.0)], auger=aK, radiat=rK, shl_num=1),
+    ])
+    E, D = dos_insulator()
+    return MaterialDesc("Al2O3-like", [al, o], dens=3.95, layer=layer, vsound=6510.0, E_F=4.4,
+                        el_eff_mass=1.0, hole_mass=1.0, cut_off=cut_off, dos_raw_E=E, dos_raw=D)
+
+
+def _al_like(layer=100.0):
+    """Al-like metal of the small CPU-runnable configuration."""
+    al = Atom(13, 26.98, 1.0, [
+        Shell(118.0, 2.0, [(130.0, 1.0, 60.0)], auger=0.47, shl_num=3),
+        Shell(73.0, 6.0, [(80.0, 1.0, 30.0), (110.0, 1.0, 60.0)], auger=11.5, shl_num=5),
+        Shell(0.1, 3.0, [(15.0, 1.0, 0.6)]),
+    ])
+    E, D = dos_free_electron(11.17)
+    return MaterialDesc("Al-like", [al], dens=2.70, layer=layer, vsound=5000.0, E_F=11.17,
+                        el_eff_mass=0.0, hole_mass=-1.0, cut_off=10.0, phonon_single_pole=True,
+                        dos_raw_E=E, dos_raw=D)
+
+
+def config_material(name):
+    """(MaterialDesc, Ion, Options, run dict) for a named configuration.
+
+    config1  Al-like metal, Xe 1 MeV/u, 100 iterations, 0-100 fs (dt 0.1)
+    config2  Al2O3-like, Au 11 MeV/u, 10^4 iterations, 0-1000 fs (dt 1)
+    test     config2's target with a reduced electron grid (2 keV), O ion at
+             0.5 MeV/u, 0-100 fs: small enough for the CPU oracle
+    """
+    opt = abi.Options()
+    ion = abi.Ion()
+    if name == "config1":
+        mat = _al_like()
+        ion.Zat, ion.Mass, ion.E = 54, 131.29, 131.29e6
+        opt.kind_of_EMFP, opt.CDF_elast_Zeff = 1, 1
+        opt.Tim, opt.dt, opt.dt_flag = 100.0, 0.1, 0
+        run = dict(nmc=100, NR=100, Rmin=1.0, Rmax=1000.0, seed=0x5EED0001)
+    elif name == "config2":
+        mat = _al2o3_like()
+        ion.Zat, ion.Mass, ion.E = 79, 196.97, 11.0e6 * 196.97
+        opt.kind_of_EMFP, opt.CDF_elast_Zeff = 0, 0
+        opt.Tim, opt.dt, opt.dt_flag = 1000.0, 1.0, 0
+        opt.include_photons = 1
+        run = dict(nmc=10000, NR=100, Rmin=1.0, Rmax=1000.0, seed=0x5EED0002)
+    elif name == "test":
+        mat = _al2o3_like()
+        ion.Zat, ion.Mass, ion.E = 8, 16.00, 0.5e6 * 16.0
+        opt.kind_of_EMFP, opt.CDF_elast_Zeff = 0, 0
+        opt.Tim, opt.dt, opt.dt_flag = 100.0, 1.0, 0
+        opt.include_photons = 1
+        opt.e_grid_Emax = 2000.0
+        run = dict(nmc=4, NR=40, Rmin=1.0, Rmax=1000.0, seed=0x5EED0003)
+    elif name == "test_metal":
+        mat = _al_like(layer=50.0)
+        ion.Zat, ion.Mass, ion.E = 8, 16.00, 0.5e6 * 16.0
+        opt.kind_of_EMFP, opt.CDF_elast_Zeff = 1, 1
+        opt.Tim, opt.dt, opt.dt_flag = 50.0, 0.5, 0
+        opt.e_grid_Emax = 2000.0
+        run = dict(nmc=2, NR=40, Rmin=1.0, Rmax=1000.0, seed=0x5EED0005)
+    else:
+        raise KeyError(name)
+    ion.Kind_Zeff, ion.fixed_Zeff, ion.Kind_ion = 0, float(ion.Zat), 0
+    opt.kind_of_DR = 1
+    return mat, ion, opt, run
