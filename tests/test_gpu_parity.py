@@ -20,9 +20,10 @@ from gpu_helpers import setup, say, sort_trace, first_mismatch, compare_tallies
 pytestmark = pytest.mark.gpu
 
 SMALL = ["test", "test_metal"]
+FULL = ["config1", "config2"]       # the Al-like/Xe and Al2O3-like/Au workloads at full size
 
 
-@pytest.mark.parametrize("name", SMALL)
+@pytest.mark.parametrize("name", SMALL + FULL)
 def test_tables_bitwise(name):
     ctx, orc, mat, ion, opt, run, R, V = setup(name, build_oracle_tables=True)
     nsh = sum(mat.n_shells)
@@ -53,10 +54,10 @@ def test_tables_bitwise(name):
     assert nrows > 100
 
 
-@pytest.mark.parametrize("name", SMALL)
+@pytest.mark.parametrize("name", SMALL + FULL)
 def test_mc_replay_bitwise(name):
     ctx, orc, mat, ion, opt, run, R, V = setup(name)
-    nmc = run["nmc"]
+    nmc = run["nmc"] if name in SMALL else 1
     say(f"[mc] {name}: GPU nmc={nmc}")
     g = ctx.run_mc(nmc, run["seed"], R, V, trace_cap=3_000_000)
     say(f"[mc] {name}: oracle")
