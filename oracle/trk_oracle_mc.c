@@ -14,7 +14,7 @@
  *
  * Random numbers: every draw comes from the Philox stream of the particle
  * that owns the event (include/trk_philox.h), in the reference's draw order
- * (Appendix A of the design notes; call sites cited inline).  Photon slots are
+ * (each random_number call site of the reference is cited inline).  Photon slots are
  * never reused (the reference overwrites live photons, MC:2844-2854 vs
  * MC:2963-2964; here each photon keeps its own slot).
  */

@@ -162,13 +162,13 @@ def test_normalised_shell_counts_electrons(lib):
 
 
 # ----------------------------------------------------------------- grids
-SURVEY_COUNTS = {  # incident-energy grids of get_grid_4CS (AI:158, 210) and the SHI grid (AI:2270-2273)
+GRID_COUNTS = {  # incident-energy grids of get_grid_4CS (AI:158, 210) and the SHI grid (AI:2270-2273)
     "config1": (476, 479, 581),
     "config2": (476, 485, 388),
 }
 
 
-@pytest.mark.parametrize("name", sorted(SURVEY_COUNTS))
+@pytest.mark.parametrize("name", sorted(GRID_COUNTS))
 def test_grid_node_counts(lib, oracle_lib, name):
     mat, ion, opt, _ = tg.config_material(name)
     mat.prepare(lib)
@@ -195,7 +195,7 @@ def test_grid_node_counts(lib, oracle_lib, name):
         else:
             assert all(have[i] for i, ip in enumerate(ips) if Emin < ip < Emax)
         got.append(n.value)
-    assert tuple(got) == SURVEY_COUNTS[name]
+    assert tuple(got) == GRID_COUNTS[name]
 
 
 # ------------------------------------------------------------------ DOS

@@ -9,8 +9,9 @@ states arrays (reading_material_DOS, RF:2374-2450) through the library, and
 builds the radial grid of Radius_for_distributions (Sorting_output_data.f90
 :1363-1435) for a caller-chosen set of radii.
 
-All choices the configurations leave open are fixed here and listed in the
-design notes (cut-off, time step, oscillator widths, DOS shapes, decay table).
+Every value a workload description leaves open (cut-off, time step,
+oscillator widths, DOS shapes, decay times) is fixed in this file, next to
+the workload that uses it.
 """
 import ctypes
 import math
