@@ -60,11 +60,14 @@ struct McDev {
     int nHel; const double *EHel, *LHel; const int64_t *roff_hel; const double *rhw_hel, *rlam_hel;
     int nP; const double *EP, *LPsh, *phot_imfp;
     int nS; const double *ES, *LSsh, *shi_path;
-    const int *sd_n; const int64_t *sd_off; const double *sd_E, *sd_L;
+    const int *sd_n; const int64_t *sd_off; const double *sd_E, *sd_L, *sd_inv; const int *sd_nonmono;
+    // logs of grids and fused tables (interpolation without per-node logs)
+    const double *lnE, *ln_el_imfp, *lnEel, *ln_Lel, *lnEH, *ln_hole_imfp, *lnEHel, *ln_LHel;
     // run
     uint64_t seed;
     double T_last, Tim, cutoff_eff;
     int Nt; const double *tim_glob;
+    double inv_dt;                      // 1/dt of a linear time grid, 0 otherwise
     int NR; const double *R;
     int include_photons;
     // lists
@@ -73,6 +76,7 @@ struct McDev {
     int n_in_pairs, n_in_phot;
     int cap_pairs, cap_phot;
     int *out_pairs, *out_phot;          // allocation counters of the next generation
+    int *cursor;                        // next item of the current generation
     int *overflow;
     unsigned long long *counters;       // [16] event counters
     double *trace; unsigned long long *trace_n; long long trace_cap;
@@ -86,6 +90,11 @@ struct PState {
     int sh;
     uint64_t id;
     uint32_t nd, ne, iter;
+    // electrons: total inelastic and elastic mean free paths at energy cE and
+    // the elastic-grid bracket of cE, kept from the previous event's free-path
+    // draw (the next event looks them up again at the same energy)
+    double cE, cIM, cEM;
+    int cN;
 };
 
 TRK_DEV double rn(const McDev &d, PState &p) { return trk_uniform(d.seed, p.id, p.iter, p.nd++); }
@@ -97,6 +106,7 @@ TRK_DEV void load(const PList &l, int i, PState &p, int hole)
     if (hole) { p.Ehkin = l.Ehkin[i]; p.Mass = l.Mass[i]; p.sh = l.sh[i]; }
     else { p.Ehkin = 0.0; p.Mass = 1.0; p.sh = -1; }
     p.id = l.id[i]; p.nd = l.nd[i]; p.ne = l.ne[i]; p.iter = l.iter[i];
+    p.cE = TRK_NAN; p.cIM = 0.0; p.cEM = 0.0; p.cN = 0;
 }
 
 TRK_DEV void store(const PList &l, int i, const PState &p, int hole)
@@ -132,6 +142,17 @@ TRK_DEV void cnt_add(unsigned long long *p, long long v) { atomicAdd(p, (unsigne
 // number of time nodes with tim_glob <= t
 TRK_DEV int nodes_le(const McDev &d, double t)
 {
+    if (d.inv_dt > 0.0) {
+        // linear grid: start next to t/dt and step to the exact count
+        if (!(t >= d.tim_glob[0])) return 0;
+        if (t >= d.tim_glob[d.Nt - 1]) return d.Nt;
+        int i = (int)(t * d.inv_dt);
+        if (i > d.Nt) i = d.Nt;
+        if (i < 0) i = 0;
+        while (i > 0 && d.tim_glob[i - 1] > t) --i;
+        while (i < d.Nt && d.tim_glob[i] <= t) ++i;
+        return i;
+    }
     int lo = 0, hi = d.Nt;
     while (lo < hi) {
         int mid = (lo + hi) >> 1;
@@ -292,6 +313,48 @@ TRK_HD double nfp_1d(const double *Ea, const double *La, int n, double E)
     int Nl = N - 1;
     if (La[Nl - 1] >= 1.0e16) return La[Nl - 1];
     return interpolate(5, Ea[Nl - 1], Ea[N - 1], La[Nl - 1], La[N - 1], E);
+}
+
+// Interpolate flag 5 with the logs of the nodes, the values and the argument
+// given: the same operations on the same numbers as interpolate(5, ...).
+TRK_HD double interp5_ln(double E1, double E2, double S1, double S2, double E, double lE1, double lE2,
+                         double lS1, double lS2, double lE)
+{
+    if (trk_fabs(E2 - E1) < 1.0e-6) return (S1 > S2) ? S1 : S2;
+    if (E == E1) return S1;
+    double o = lS1 + ((lS2 - lS1) / (lE2 - lE1)) * (lE - lE1);
+    return trk_exp(o);
+}
+
+// nfp_2d / nfp_1d on a table with precomputed logs (lEa, lLa); lE = trk_log(E).
+// *Nout receives the bracket index of the search.
+TRK_HD double nfp_2d_ln(const double *Ea, const double *La, const double *lEa, const double *lLa, int n, double E,
+                        double lE, int *Nout)
+{
+    int N = find_mono_2d(Ea, n, E);
+    *Nout = N;
+    if (N == 1) {
+        double M = interpolate(1, Ea[0], Ea[1], La[0], La[1], E);
+        return (M < La[0]) ? La[0] : M;
+    }
+    int Nl = N - 1;
+    if (La[Nl - 1] >= 1.0e16) return La[Nl - 1];
+    return interp5_ln(Ea[Nl - 1], Ea[N - 1], La[Nl - 1], La[N - 1], E, lEa[Nl - 1], lEa[N - 1], lLa[Nl - 1],
+                      lLa[N - 1], lE);
+}
+
+TRK_HD double nfp_1d_ln(const double *Ea, const double *La, const double *lEa, const double *lLa, int n, double E,
+                        double lE)
+{
+    int N = find_mono_1d(Ea, n, E);
+    if (N == 1) {
+        double M = interpolate(1, Ea[0], Ea[1], La[0], La[1], E);
+        return (M < Ea[0]) ? Ea[0] : M;
+    }
+    int Nl = N - 1;
+    if (La[Nl - 1] >= 1.0e16) return La[Nl - 1];
+    return interp5_ln(Ea[Nl - 1], Ea[N - 1], La[Nl - 1], La[N - 1], E, lEa[Nl - 1], lEa[N - 1], lLa[Nl - 1],
+                      lLa[N - 1], lE);
 }
 
 // Which_shell, MC:1786-1832 (per-shell lambda on a shared grid)
@@ -593,8 +656,22 @@ TRK_DEV void hole_params(const McDev &d, PState &h, double Eh, PState &o)
 
 TRK_DEV double el_mfp(const McDev &d, double E, double RN)
 {
-    double IM = nfp_2d(d.E, d.el_imfp, d.nE, E);
-    double EM = nfp_2d(d.Eel, d.Lel, d.nEel, E);
+    double lE = trk_log(E);
+    int N;
+    double IM = nfp_2d_ln(d.E, d.el_imfp, d.lnE, d.ln_el_imfp, d.nE, E, lE, &N);
+    double EM = nfp_2d_ln(d.Eel, d.Lel, d.lnEel, d.ln_Lel, d.nEel, E, lE, &N);
+    return -trk_log(RN) / (1.0 / IM + 1.0 / EM);
+}
+
+// el_mfp for the electron's own next flight: also keeps both mean free paths
+// and the elastic bracket for the electron's next event at this energy.
+TRK_DEV double el_mfp_keep(const McDev &d, double E, double RN, PState &e)
+{
+    double lE = trk_log(E);
+    int N;
+    double IM = nfp_2d_ln(d.E, d.el_imfp, d.lnE, d.ln_el_imfp, d.nE, E, lE, &N);
+    double EM = nfp_2d_ln(d.Eel, d.Lel, d.lnEel, d.ln_Lel, d.nEel, E, lE, &N);
+    e.cE = E; e.cIM = IM; e.cEM = EM; e.cN = N;
     return -trk_log(RN) / (1.0 / IM + 1.0 / EM);
 }
 
@@ -639,8 +716,15 @@ TRK_DEV int electron_event(const McDev &d, PState &e, int rep)
     uint32_t ev = e.ne++;
     double t_ev = e.tn;
     double Eel = e.E;
-    double IMFP = nfp_2d(d.E, d.el_imfp, d.nE, Eel);
-    double EMFP = nfp_2d(d.Eel, d.Lel, d.nEel, Eel);
+    double IMFP, EMFP;
+    int Nel;
+    if (Eel == e.cE) {
+        IMFP = e.cIM; EMFP = e.cEM; Nel = e.cN;
+    } else {
+        double lE = trk_log(Eel);
+        IMFP = nfp_2d_ln(d.E, d.el_imfp, d.lnE, d.ln_el_imfp, d.nE, Eel, lE, &Nel);
+        EMFP = nfp_2d_ln(d.Eel, d.Lel, d.lnEel, d.ln_Lel, d.nEel, Eel, lE, &Nel);
+    }
     double RN = rn(d, e);
     double L = e.L, th0 = e.th, ph0 = e.ph;
     double X = e.X + (L * trk_sin(th0)) * trk_sin(ph0);
@@ -674,7 +758,11 @@ TRK_DEV int electron_event(const McDev &d, PState &e, int rep)
         emit_pair(d, ne, nh);
     } else {
         kind = EV_E_EL;
-        double EMt = nfp_1d(d.Eel, d.Lel, d.nEel, Eel);
+        // nfp_1d brackets like nfp_2d unless the energy sits on a node or
+        // outside the grid; only then is the lookup repeated
+        double EMt;
+        if (Nel > 1 && d.Eel[Nel - 2] < Eel && Eel < d.Eel[Nel - 1]) EMt = EMFP;
+        else EMt = nfp_1d_ln(d.Eel, d.Lel, d.lnEel, d.ln_Lel, d.nEel, Eel, trk_log(Eel));
         if (d.m.kind_of_EMFP == 1) {
             dE = transfer_el(d, Eel, EMt, 0, e);
         } else {
@@ -687,7 +775,7 @@ TRK_DEV int electron_event(const McDev &d, PState &e, int rep)
         phi = (2.0 * kPi) * rn(d, e);
         tally_elat(d, rep, t_ev, X, Y, dE);
     }
-    double MFP = el_mfp(d, Eel - dE, rn(d, e));
+    double MFP = el_mfp_keep(d, Eel - dE, rn(d, e), e);
     double phi1, theta1;
     new_angles(ph0, th0, theta, phi, phi1, theta1);
     e.E = Eel - dE; e.t0 = e.tn; e.X = X; e.Y = Y; e.Z = Z; e.L = MFP; e.th = theta1; e.ph = phi1;
@@ -916,9 +1004,24 @@ TRK_DEV double shi_transfer(const McDev &d, int shell, PState &I)
     double Tot = ((dL > 0.0) && (L[N - 1] > 0.0)) ? 1.0 / dL + RN * (1.0 / L[N - 1] - 1.0 / dL) : 1.5e21;
     int Nt;
     if (Tot < 1.0e20) {
-        int i = 1;
-        while (i < N && 1.0 / L[i - 1] < Tot) i++;
-        Nt = i;
+        // SHI_energy_transfer scans linearly for the first i with
+        // 1/L(i) >= Tot among i < N (Find_in_1D_array, RF:3348-3358); on a
+        // row whose inverse never decreases a lower-bound bisection finds the
+        // same index in log2(N) steps (the rows have ~10^3-10^4 nodes).
+        const double *inv = d.sd_inv + d.sd_off[shell];
+        if (!d.sd_nonmono[shell]) {
+            int lo = 0, hi = N - 1;
+            while (lo < hi) {
+                int mid = (lo + hi) >> 1;
+                if (inv[mid] < Tot) lo = mid + 1;
+                else hi = mid;
+            }
+            Nt = lo + 1;
+        } else {
+            int i = 1;
+            while (i < N && inv[i - 1] < Tot) i++;
+            Nt = i;
+        }
     } else {
         Nt = M;
     }
@@ -1014,25 +1117,34 @@ __global__ void __launch_bounds__(64) k_shi(McDev d, int nb, uint32_t first_iter
     if (c_shi) atomicAdd(&d.counters[EV_SHI], c_shi);
 }
 
-// One thread per particle of the generation: electrons [0, np), holes
-// [np, 2np), photons [2np, 2np+nq).
+// Transport of one generation: items [0, np) are electrons, [np, 2np) holes,
+// [2np, 2np+nq) photons.  A lane takes the next item from a shared cursor
+// whenever its particle has finished (below the cut-off, absorbed, or past
+// the last time node), so a wave stays busy until the generation is used up
+// instead of waiting for the longest of its 64 initial histories.  Every lane
+// leaves once the cursor has passed the last item.
 __global__ void __launch_bounds__(256) k_transport(McDev d)
 {
-    int t = blockIdx.x * blockDim.x + threadIdx.x;
     const int np = d.n_in_pairs, nq = d.n_in_phot;
-    if (t >= 2 * np + nq) return;
+    const int total = 2 * np + nq;
     unsigned long long c_ei = 0, c_ee = 0, c_hi = 0, c_he = 0, c_au = 0, c_ra = 0, c_ph = 0;
     PState p;
-    int kind;
-    if (t < np) { kind = 0; load(d.in_e, t, p, 0); }
-    else if (t < 2 * np) { kind = 1; load(d.in_h, t - np, p, 1); }
-    else { kind = 2; load(d.in_p, t - 2 * np, p, 0); }
-    const int rep = (int)(p.iter % (uint32_t)d.T.R);
+    int kind = 0, rep = 0;
+    bool have = false;
     for (;;) {
+        if (!have) {
+            int t = wave_alloc(d.cursor);
+            if (t >= total) break;
+            if (t < np) { kind = 0; load(d.in_e, t, p, 0); }
+            else if (t < 2 * np) { kind = 1; load(d.in_h, t - np, p, 1); }
+            else { kind = 2; load(d.in_p, t - 2 * np, p, 0); }
+            rep = (int)(p.iter % (uint32_t)d.T.R);
+            have = true;
+        }
         segment(d, p, kind, rep);
-        if (!(p.tn < d.T_last)) break;
-        if (p.ne >= kMaxEvents) { atomicOr(d.overflow, kFlagEvents); break; }
-        if (p.E != p.E) { atomicOr(d.overflow, kFlagNaN); break; }
+        if (!(p.tn < d.T_last)) { have = false; continue; }
+        if (p.ne >= kMaxEvents) { atomicOr(d.overflow, kFlagEvents); have = false; continue; }
+        if (p.E != p.E) { atomicOr(d.overflow, kFlagNaN); have = false; continue; }
         if (kind == 0) {
             int k = electron_event(d, p, rep);
             if (k == EV_E_INEL) c_ei++; else c_ee++;
@@ -1045,7 +1157,7 @@ __global__ void __launch_bounds__(256) k_transport(McDev d)
         } else {
             photon_event(d, p);
             c_ph++;
-            break;
+            have = false;
         }
     }
     if (c_ei) atomicAdd(&d.counters[EV_E_INEL], c_ei);
@@ -1061,7 +1173,8 @@ __global__ void __launch_bounds__(256) k_transport(McDev d)
 struct McState {
     PList L[2][3];            // [ping-pong][e, h, p]
     int cap_pairs = 0, cap_phot = 0;
-    int *d_counts = nullptr;  // [4]: pairs A, phot A, pairs B, phot B
+    int *d_counts = nullptr;  // [8]: pairs A, phot A, pairs B, phot B, transport cursor
+    int64_t max_blocks = 0;   // resident transport workgroups on the device
     int *d_overflow = nullptr;
     unsigned long long *d_counters = nullptr;
     Tally T{};
@@ -1198,6 +1311,12 @@ int run_mc(trk_ctx *c, int64_t nmc, uint64_t seed, int64_t first_iter, const dou
         if (!s->d_overflow) HIPC(hipMalloc((void **)&s->d_overflow, sizeof(int)));
         if (!s->d_counters) HIPC(hipMalloc((void **)&s->d_counters, 16 * sizeof(unsigned long long)));
         if (!s->ev0) { HIPC(hipEventCreate(&s->ev0)); HIPC(hipEventCreate(&s->ev1)); }
+        int per_cu = 0, dev = 0, ncu = 0;
+        HIPC(hipGetDevice(&dev));
+        HIPC(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+        HIPC(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_transport, 256, 0));
+        if (per_cu < 1) per_cu = 1;
+        s->max_blocks = (int64_t)per_cu * ncu;
     }
     if (ensure_tally(c, s, Nt, NR, nsh, R)) return TRK_ERR_DEVICE;
     if (s->d_tim) (void)hipFree(s->d_tim);
@@ -1231,12 +1350,20 @@ int run_mc(trk_ctx *c, int64_t nmc, uint64_t seed, int64_t first_iter, const dou
 
     McDev d{};
     fill_mcdev(c, d);
-    d.sd_n = d_sdn; d.sd_off = d_sdoff; d.sd_E = td.dE; d.sd_L = td.dL;
+    d.sd_n = d_sdn; d.sd_off = d_sdoff; d.sd_E = td.dE; d.sd_L = td.dL; d.sd_inv = td.dInv; d.sd_nonmono = td.dNonMono;
     d.seed = seed;
     d.Tim = h.o.Tim;
     d.T_last = h.tim_glob[Nt - 1];
     d.cutoff_eff = (h.m.cut_off > 0.0) ? h.m.cut_off : 0.0;
     d.Nt = Nt; d.tim_glob = s->d_tim;
+    d.inv_dt = (h.o.dt_flag <= 0 && h.o.dt > 0.0) ? 1.0 / h.o.dt : 0.0;
+    {
+        const TableSet &ts = c->ts;
+        d.lnE = ts.ln_E[TRK_TAB_E_INEL]; d.ln_el_imfp = ts.ln_el_imfp;
+        d.lnEel = ts.ln_E[TRK_TAB_E_EL]; d.ln_Lel = ts.ln_Lel;
+        d.lnEH = ts.ln_E[TRK_TAB_H_INEL]; d.ln_hole_imfp = ts.ln_hole_imfp;
+        d.lnEHel = ts.ln_E[TRK_TAB_H_EL]; d.ln_LHel = ts.ln_LHel;
+    }
     d.NR = NR; d.R = s->d_R;
     d.include_photons = h.o.include_photons;
     d.cap_pairs = s->cap_pairs; d.cap_phot = s->cap_phot;
@@ -1312,8 +1439,12 @@ int run_mc(trk_ctx *c, int64_t nmc, uint64_t seed, int64_t first_iter, const dou
             d.out_e = s->L[nxt][0]; d.out_h = s->L[nxt][1]; d.out_p = s->L[nxt][2];
             d.out_pairs = s->d_counts + 2 * nxt; d.out_phot = s->d_counts + 2 * nxt + 1;
             int64_t nthr = 2 * (int64_t)np + nq;
+            int64_t nblk = (nthr + 255) / 256;
+            if (nblk > s->max_blocks) nblk = s->max_blocks;
+            d.cursor = s->d_counts + 4;
+            HIPC(hipMemsetAsync(d.cursor, 0, sizeof(int), st));
             if (c->prof) HIPC(hipEventRecord(c->pe[0], st));
-            k_transport<<<(unsigned)((nthr + 255) / 256), 256, 0, st>>>(d);
+            k_transport<<<(unsigned)nblk, 256, 0, st>>>(d);
             HIPC(hipGetLastError());
             if (c->prof) { HIPC(hipEventRecord(c->pe[1], st)); pending = 1; }
             launches++;

@@ -49,6 +49,8 @@ struct TableDev {
     std::vector<int64_t> doff;
     double *dE = nullptr;
     double *dL = nullptr;
+    double *dInv = nullptr;           // 1/dL, for the ion's transfer-energy search
+    int *dNonMono = nullptr;          // [nsh] nonzero if 1/dL of the shell's row ever decreases
 };
 
 struct TableSet {
@@ -57,6 +59,11 @@ struct TableSet {
     double *hole_imfp = nullptr;
     double *phot_imfp = nullptr;
     double *shi_path = nullptr;
+    // trk_log of every grid node and of every fused / electron-elastic / hole-
+    // elastic mean free path: the log-log interpolation of a lookup then
+    // needs only the log of the particle energy (same values, same result)
+    double *ln_E[7] = {};          // per kind, [n]
+    double *ln_el_imfp = nullptr, *ln_hole_imfp = nullptr, *ln_Lel = nullptr, *ln_LHel = nullptr;
     int64_t launches = 0;
     double build_seconds = 0.0;
     int64_t dcs_evals = 0;         // number of q-integrals evaluated

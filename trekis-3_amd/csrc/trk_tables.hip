@@ -286,6 +286,26 @@ __global__ void k_eval(MatDev m, const RowJob *__restrict__ jobs, int nrows, int
     else fb[k] = eval_point(m, j, E + dE);
 }
 
+// inv[k] = 1/L[k] for one row of the ion's transfer table; *nonmono is set
+// if the inverse ever decreases (or is NaN) along the row.
+__global__ void k_inv_row(const double *__restrict__ L, int n, double *__restrict__ inv, int *nonmono)
+{
+    int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    double a = 1.0 / L[k];
+    inv[k] = a;
+    if (k + 1 < n) {
+        double b = 1.0 / L[k + 1];
+        if (!(a <= b)) atomicOr(nonmono, 1);
+    }
+}
+
+__global__ void k_log(const double *__restrict__ x, int n, double *__restrict__ out)
+{
+    int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < n) out[k] = trk_log(x[k]);
+}
+
 struct RowOut {
     double *L, *dEdx;        // table arrays for the row's kind (shell-major)
     int n;                   // table grid size
@@ -606,6 +626,8 @@ void free_tables(TableSet &ts)
         if (t.lam) (void)hipFree(t.lam);
         if (t.dE) (void)hipFree(t.dE);
         if (t.dL) (void)hipFree(t.dL);
+        if (t.dInv) (void)hipFree(t.dInv);
+        if (t.dNonMono) (void)hipFree(t.dNonMono);
         t = TableDev();
     }
     if (ts.el_imfp) (void)hipFree(ts.el_imfp);
@@ -613,6 +635,9 @@ void free_tables(TableSet &ts)
     if (ts.phot_imfp) (void)hipFree(ts.phot_imfp);
     if (ts.shi_path) (void)hipFree(ts.shi_path);
     ts.el_imfp = ts.hole_imfp = ts.phot_imfp = ts.shi_path = nullptr;
+    for (double *&p : ts.ln_E) { if (p) (void)hipFree(p); p = nullptr; }
+    double **lp[] = {&ts.ln_el_imfp, &ts.ln_hole_imfp, &ts.ln_Lel, &ts.ln_LHel};
+    for (double **p : lp) { if (*p) (void)hipFree(*p); *p = nullptr; }
 }
 
 static int upload_grid(TableDev &t, const std::vector<double> &g, int nsh, std::string &err)
@@ -764,6 +789,15 @@ int build_tables(const HostMat &h, const MatDev &md, TableSet &ts, hipStream_t s
                                                 td.dE, td.dL, d_doff);
     HIP_OK(hipGetLastError());
 
+    // inverse rows of the ion's transfer table and their monotonicity
+    if (dalloc(&td.dInv, td.doff[nsh] > 0 ? td.doff[nsh] : 1, err) || dalloc(&td.dNonMono, nsh, err))
+        return TRK_ERR_DEVICE;
+    HIP_OK(hipMemsetAsync(td.dNonMono, 0, sizeof(int) * nsh, st));
+    for (int s = 0; s < nsh; ++s)
+        if (dn[s] > 0)
+            k_inv_row<<<nblk(dn[s], 256), 256, 0, st>>>(td.dL + td.doff[s], dn[s], td.dInv + td.doff[s], td.dNonMono + s);
+    HIP_OK(hipGetLastError());
+
     // Mott / disabled elastic tables
     if (h.o.kind_of_EMFP != 1) {
         k_mott<<<nblk(T[TRK_TAB_E_EL].n, 128), 128, 0, st>>>(md, T[TRK_TAB_E_EL].E, T[TRK_TAB_E_EL].n, 0,
@@ -787,6 +821,21 @@ int build_tables(const HostMat &h, const MatDev &md, TableSet &ts, hipStream_t s
     if (!gP.empty()) {
         if (dalloc(&ts.phot_imfp, T[TRK_TAB_PHOT].n, err)) return TRK_ERR_DEVICE;
         k_fuse<<<nblk(T[TRK_TAB_PHOT].n, 128), 128, 0, st>>>(T[TRK_TAB_PHOT].L, T[TRK_TAB_PHOT].n, nsh, 0, ts.phot_imfp);
+    }
+    HIP_OK(hipGetLastError());
+    // logs of the grids and of the tables the transport interpolates most
+    for (int k = 0; k < 6; ++k) {
+        if (T[k].n <= 0 || !T[k].E) continue;
+        if (dalloc(&ts.ln_E[k], T[k].n, err)) return TRK_ERR_DEVICE;
+        k_log<<<nblk(T[k].n, 128), 128, 0, st>>>(T[k].E, T[k].n, ts.ln_E[k]);
+    }
+    struct { const double *src; int n; double **dst; } lg[] = {
+        {ts.el_imfp, T[TRK_TAB_E_INEL].n, &ts.ln_el_imfp}, {ts.hole_imfp, T[TRK_TAB_H_INEL].n, &ts.ln_hole_imfp},
+        {T[TRK_TAB_E_EL].L, T[TRK_TAB_E_EL].n, &ts.ln_Lel}, {T[TRK_TAB_H_EL].L, T[TRK_TAB_H_EL].n, &ts.ln_LHel}};
+    for (auto &g : lg) {
+        if (!g.src || g.n <= 0) continue;
+        if (dalloc(g.dst, g.n, err)) return TRK_ERR_DEVICE;
+        k_log<<<nblk(g.n, 128), 128, 0, st>>>(g.src, g.n, *g.dst);
     }
     HIP_OK(hipGetLastError());
     HIP_OK(hipStreamSynchronize(st));
