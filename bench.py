@@ -60,7 +60,20 @@ def main():
     ap.add_argument("--cpu-iters", type=int, default=1, help="oracle iterations for the CPU baseline (0 = skip)")
     ap.add_argument("--config", default="config2")
     ap.add_argument("--gen-capacity", type=int, default=1 << 24, help="particle pairs per generation buffer")
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
+                    help="only run a plain device-to-device copy of MIB MiB five times (the reference kernel "
+                         "that scales HBM byte counters) and exit")
     args = ap.parse_args()
+    if args.calibrate_copy:
+        import torch
+        n = args.calibrate_copy * (1 << 20) // 4
+        x = torch.ones(n, dtype=torch.float32, device="cuda")
+        y = torch.empty_like(x)
+        for _ in range(5):
+            y.copy_(x)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes_read": n * 4, "copy_bytes_written": n * 4, "launches": 5}), flush=True)
+        return
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); the multi-GPU split of "
               f"iterations is not built in this tree", file=sys.stderr)
@@ -145,7 +158,7 @@ def main():
         "config": {"workload": f"{mat.name} target, Au 11 MeV/u ion, 0-1000 fs x 100 radial bins, "
                                f"{B} iterations per step", "iters_per_step": B, "events": total_ev,
                    "events_by_kind": dict(zip(abi.EVENT_NAMES, ev.tolist())), "table_build_s": t_tables,
-                   "shi_kernel_ms": ms_shi, "shi_launches": n_shi,
+                   "ion_init_kernel_ms": ms_shi, "ion_init_launches": n_shi,
                    "outside_batches_ms_per_step": sum(host_ms) / len(host_ms)},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None,

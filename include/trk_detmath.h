@@ -182,17 +182,17 @@ TRK_FN double trk_reduce_pio2(double x, int *q)
     return r;
 }
 
+/* Both polynomials are always evaluated and the quadrant picks one: on a GPU
+ * the lanes of a wave sit in different quadrants, and a branch per quadrant
+ * would run every path anyway. */
 TRK_FN double trk_sin(double x)
 {
     if (x != x || trk_fabs(x) == TRK_INF) return TRK_NAN;
     int q;
     double r = trk_reduce_pio2(x, &q);
-    switch (q) {
-    case 0: return trk_sin_poly(r);
-    case 1: return trk_cos_poly(r);
-    case 2: return -trk_sin_poly(r);
-    default: return -trk_cos_poly(r);
-    }
+    double sp = trk_sin_poly(r), cp = trk_cos_poly(r);
+    double v = (q & 1) ? cp : sp;
+    return (q & 2) ? -v : v;
 }
 
 TRK_FN double trk_cos(double x)
@@ -200,12 +200,23 @@ TRK_FN double trk_cos(double x)
     if (x != x || trk_fabs(x) == TRK_INF) return TRK_NAN;
     int q;
     double r = trk_reduce_pio2(x, &q);
-    switch (q) {
-    case 0: return trk_cos_poly(r);
-    case 1: return -trk_sin_poly(r);
-    case 2: return -trk_cos_poly(r);
-    default: return trk_sin_poly(r);
-    }
+    double sp = trk_sin_poly(r), cp = trk_cos_poly(r);
+    double v = (q & 1) ? sp : cp;
+    return ((q + 1) & 2) ? -v : v;
+}
+
+/* sin and cos of one argument from one reduction; each equals trk_sin(x) /
+ * trk_cos(x) bit for bit. */
+TRK_FN void trk_sincos(double x, double *s, double *c)
+{
+    if (x != x || trk_fabs(x) == TRK_INF) { *s = TRK_NAN; *c = TRK_NAN; return; }
+    int q;
+    double r = trk_reduce_pio2(x, &q);
+    double sp = trk_sin_poly(r), cp = trk_cos_poly(r);
+    double vs = (q & 1) ? cp : sp;
+    double vc = (q & 1) ? sp : cp;
+    *s = (q & 2) ? -vs : vs;
+    *c = ((q + 1) & 2) ? -vc : vc;
 }
 
 /* --- atan / acos ------------------------------------------------------------ */
@@ -237,9 +248,9 @@ TRK_FN double trk_atan(double x)
     const double PIO2_LO = 0x1.1a62633145c07p-54;
     if (x != x) return x;
     double ax = trk_fabs(x);
-    double r;
-    if (ax > 1.0) r = (PIO2 - trk_atan_small(1.0 / ax)) + PIO2_LO;
-    else r = trk_atan_small(ax);
+    const int big = ax > 1.0;
+    double a = trk_atan_small(big ? 1.0 / ax : ax);   /* one evaluation for every lane */
+    double r = big ? (PIO2 - a) + PIO2_LO : a;
     return (x < 0.0) ? -r : r;
 }
 

@@ -27,6 +27,8 @@
 //    of the order of the adds and hence deterministic.
 #include <hip/hip_runtime.h>
 #include <chrono>
+#include <cstdio>
+#include <cstdlib>
 #include <cmath>
 #include <cstring>
 #include <string>
@@ -68,15 +70,17 @@ struct McDev {
     double T_last, Tim, cutoff_eff;
     int Nt; const double *tim_glob;
     double inv_dt;                      // 1/dt of a linear time grid, 0 otherwise
+    // tables copied into each transport workgroup's LDS (count and offset in
+    // doubles; count 0 = read from global memory), see kStage* below
+    int lds_n[24], lds_off[24];
     int NR; const double *R;
     int include_photons;
-    // lists
-    PList in_e, in_h, in_p;
-    PList out_e, out_h, out_p;
-    int n_in_pairs, n_in_phot;
-    int cap_pairs, cap_phot;
-    int *out_pairs, *out_phot;          // allocation counters of the next generation
-    int *cursor;                        // next item of the current generation
+    // lists, by kind: electrons, holes, photons, ions
+    PList in[4], out[4];
+    int n_in[4];                        // particles of the current round
+    int cap[4];                         // capacity of each list
+    int *out_cnt;                       // [4] allocation counters of the next round
+    int *cursor;                        // next item of the current round
     int *overflow;
     unsigned long long *counters;       // [16] event counters
     double *trace; unsigned long long *trace_n; long long trace_cap;
@@ -233,8 +237,8 @@ TRK_DEV void segment(const McDev &d, const PState &p, int kind, int rep)
         moving = true;
         V = kC;
     }
-    if (moving) { st = trk_sin(p.th); sp = trk_sin(p.ph); cp = trk_cos(p.ph); }
-    else { st = trk_sin(0.0); sp = trk_sin(0.0); cp = trk_cos(0.0); }
+    if (moving) { st = trk_sin(p.th); trk_sincos(p.ph, &sp, &cp); }
+    else { st = trk_sin(0.0); trk_sincos(0.0, &sp, &cp); }
     int jprev = -1, rstart = r0;
     for (int r = r0; r <= r1; ++r) {
         double L0 = 0.0;
@@ -533,10 +537,13 @@ TRK_DEV double cos_theta_W(const McDev &d, double E, double W, double M_in, PSta
 }
 
 // New_Angles_both, MC:1328-1360
-TRK_DEV void new_angles(double phi0, double theta0, double theta, double psi, double &phi1, double &theta1)
+// ct0 = trk_cos(theta0), computed by the caller for the flight before.
+TRK_DEV void new_angles(double phi0, double theta0, double ct0, double theta, double psi, double &phi1, double &theta1)
 {
-    double ph = phi0 + ((theta * trk_cos(theta0)) * trk_sin(psi));
-    double t = theta0 + (theta * trk_cos(psi));
+    double sps, cps;
+    trk_sincos(psi, &sps, &cps);
+    double ph = phi0 + ((theta * ct0) * sps);
+    double t = theta0 + (theta * cps);
     while (t < 0.0) { t = trk_fabs(t); ph = ph + kPi; }
     while (t > kPi) { t = (2.0 * kPi) - t; ph = ph - kPi; }
     if (ph > 2.0 * kPi) ph = ph - (((double)floor(ph / (2.0 * kPi))) * 2.0) * kPi;
@@ -682,6 +689,8 @@ constexpr int kFlagPairs = 1, kFlagPhotons = 2, kFlagEvents = 4, kFlagNaN = 8;
 // thousand at most; the limit only guarantees that every wave ends.
 constexpr uint32_t kMaxEvents = 1u << 22;
 
+// Slot allocation aggregated over the active lanes of a wave: one atomic per
+// wave.  Every active lane must pass the same counter.
 TRK_DEV int wave_alloc(int *counter)
 {
     unsigned long long mask = __ballot(1);
@@ -694,20 +703,23 @@ TRK_DEV int wave_alloc(int *counter)
     return base + rank;
 }
 
-TRK_DEV void emit_pair(const McDev &d, const PState &e, const PState &h)
+enum { K_EL = 0, K_HO = 1, K_PH = 2, K_ION = 3 };
+
+// Append one particle to the next round's list of its kind.
+TRK_DEV void emit(const McDev &d, int kind, const PState &q)
 {
-    int k = wave_alloc(d.out_pairs);
-    if (k >= d.cap_pairs) { atomicOr(d.overflow, kFlagPairs); return; }
-    store(d.out_e, k, e, 0);
-    store(d.out_h, k, h, 1);
+    int k = wave_alloc(d.out_cnt + kind);
+    if (k >= d.cap[kind]) { atomicOr(d.overflow, kind == K_PH ? kFlagPhotons : kFlagPairs); return; }
+    store(d.out[kind], k, q, kind == K_HO || kind == K_ION);
 }
 
-TRK_DEV void emit_photon(const McDev &d, const PState &q)
+TRK_DEV void emit_pair(const McDev &d, const PState &e, const PState &h)
 {
-    int k = wave_alloc(d.out_phot);
-    if (k >= d.cap_phot) { atomicOr(d.overflow, kFlagPhotons); return; }
-    store(d.out_p, k, q, 0);
+    emit(d, K_EL, e);
+    emit(d, K_HO, h);
 }
+
+TRK_DEV void emit_photon(const McDev &d, const PState &q) { emit(d, K_PH, q); }
 
 // ---------------------------------------------------------------- events
 // Electron_Monte_Carlo, MC:2253-2474
@@ -727,9 +739,12 @@ TRK_DEV int electron_event(const McDev &d, PState &e, int rep)
     }
     double RN = rn(d, e);
     double L = e.L, th0 = e.th, ph0 = e.ph;
-    double X = e.X + (L * trk_sin(th0)) * trk_sin(ph0);
-    double Y = e.Y + (L * trk_sin(th0)) * trk_cos(ph0);
-    double Z = e.Z + L * trk_cos(th0);
+    double sth0, cth0, sph0, cph0;
+    trk_sincos(th0, &sth0, &cth0);
+    trk_sincos(ph0, &sph0, &cph0);
+    double X = e.X + (L * sth0) * sph0;
+    double Y = e.Y + (L * sth0) * cph0;
+    double Z = e.Z + L * cth0;
     double dE, theta, phi;
     int kind;
     if (RN * (1.0 / IMFP + 1.0 / EMFP) < 1.0 / IMFP) {
@@ -743,7 +758,7 @@ TRK_DEV int electron_event(const McDev &d, PState &e, int rep)
         double dE_cur = receives_E(d, dE, sh, e);
         double MFP = el_mfp(d, dE_cur, rn(d, e));
         double phi1, theta1;
-        new_angles(ph0, th0, kPi / 2.0 - theta, phi + kPi, phi1, theta1);
+        new_angles(ph0, th0, cth0, kPi / 2.0 - theta, phi + kPi, phi1, theta1);
         PState ne, nh;
         init_child(ne, e.id, ev, 0, e.iter, 0);
         init_child(nh, e.id, ev, 1, e.iter, 1);
@@ -777,7 +792,7 @@ TRK_DEV int electron_event(const McDev &d, PState &e, int rep)
     }
     double MFP = el_mfp_keep(d, Eel - dE, rn(d, e), e);
     double phi1, theta1;
-    new_angles(ph0, th0, theta, phi, phi1, theta1);
+    new_angles(ph0, th0, cth0, theta, phi, phi1, theta1);
     e.E = Eel - dE; e.t0 = e.tn; e.X = X; e.Y = Y; e.Z = Z; e.L = MFP; e.th = theta1; e.ph = phi1;
     e.tn = t_next(e.t0, MFP, vel_el(e.E));
     cut_el(d, e);
@@ -815,13 +830,23 @@ TRK_DEV int hole_event(const McDev &d, PState &h, int rep)
     int kind;
     if (h.sh == d.m.vb) {
         double Eel = h.Ehkin;
-        double HI = nfp_2d(d.EH, d.hole_imfp, d.nH, Eel);
-        double HE = nfp_2d(d.EHel, d.LHel, d.nHel, Eel);
+        double HI, HE;
+        if (Eel == h.cE) {
+            HI = h.cIM; HE = h.cEM;
+        } else {
+            double lE = trk_log(Eel);
+            int N;
+            HI = nfp_2d_ln(d.EH, d.hole_imfp, d.lnEH, d.ln_hole_imfp, d.nH, Eel, lE, &N);
+            HE = nfp_2d_ln(d.EHel, d.LHel, d.lnEHel, d.ln_LHel, d.nHel, Eel, lE, &N);
+        }
         double RN = rn(d, h);
         double L = h.L, th0 = h.th, ph0 = h.ph;
-        double X = h.X + (L * trk_sin(th0)) * trk_sin(ph0);
-        double Y = h.Y + (L * trk_sin(th0)) * trk_cos(ph0);
-        double Z = h.Z + L * trk_cos(th0);
+        double sth0, cth0, sph0, cph0;
+        trk_sincos(th0, &sth0, &cth0);
+        trk_sincos(ph0, &sph0, &cph0);
+        double X = h.X + (L * sth0) * sph0;
+        double Y = h.Y + (L * sth0) * cph0;
+        double Z = h.Z + L * cth0;
         double dE, Ehole, hth1, hph1;
         if ((RN * (1.0 / HI + 1.0 / HE) < 1.0 / HI) && (HI < 1.0e15)) {
             kind = EV_H_INEL;
@@ -840,7 +865,7 @@ TRK_DEV int hole_event(const McDev &d, PState &h, int rep)
             double MFP = el_mfp(d, dE_cur, rn(d, h));
             (void)rn(d, h);  // drawn and discarded, MC:2611
             double phi1, theta1;
-            new_angles(ph0, th0, hth, hph, phi1, theta1);
+            new_angles(ph0, th0, cth0, hth, hph, phi1, theta1);
             PState ne, nh;
             init_child(ne, h.id, ev, 0, h.iter, 0);
             init_child(nh, h.id, ev, 1, h.iter, 1);
@@ -871,7 +896,7 @@ TRK_DEV int hole_event(const McDev &d, PState &h, int rep)
             tally_elat(d, rep, t_ev, X, Y, dE);
         }
         double hph2, hth2;
-        new_angles(ph0, th0, hth1, hph1, hph2, hth2);
+        new_angles(ph0, th0, cth0, hth1, hph1, hph2, hth2);
         h.t0 = h.tn; h.X = X; h.Y = Y; h.Z = Z; h.th = hth2; h.ph = hph2;
         hole_params(d, h, Ehole + Egap, h);
         cut_ho(d, h);
@@ -963,14 +988,17 @@ TRK_DEV void photon_event(const McDev &d, PState &q)
     double t_ev = q.tn;
     double Eel = q.E;
     double L = q.L, th0 = q.th, ph0 = q.ph;
-    double X = q.X + (L * trk_sin(th0)) * trk_sin(ph0);
-    double Y = q.Y + (L * trk_sin(th0)) * trk_cos(ph0);
-    double Z = q.Z + L * trk_cos(th0);
+    double sth0, cth0, sph0, cph0;
+    trk_sincos(th0, &sth0, &cth0);
+    trk_sincos(ph0, &sph0, &cph0);
+    double X = q.X + (L * sth0) * sph0;
+    double Y = q.Y + (L * sth0) * cph0;
+    double Z = q.Z + L * cth0;
     int sh = which_shell(d, d.EP, d.LPsh, d.nP, Eel, q);
     double dE_cur = receives_E(d, Eel, sh, q);
     double MFP = el_mfp(d, dE_cur, rn(d, q));
     double phi1, theta1;
-    new_angles(q.ph, q.th, kPi / 2.0, 0.0, phi1, theta1);
+    new_angles(q.ph, q.th, cth0, kPi / 2.0, 0.0, phi1, theta1);
     PState ne, nh;
     init_child(ne, q.id, ev, 0, q.iter, 0);
     init_child(nh, q.id, ev, 1, q.iter, 1);
@@ -1055,13 +1083,12 @@ TRK_DEV double shi_zeff_dev(const McDev &d, double E)
     }
 }
 
-// One thread per iteration: the ion's chain of ionisations (SHI_Monte_Carlo,
-// MC:2153-2249, driven by the time loop MC:563-643).
-__global__ void __launch_bounds__(64) k_shi(McDev d, int nb, uint32_t first_iter)
+// The ion of every iteration starts at the surface with its first free
+// flight drawn (MC:564); one thread per iteration writes it to the ion list.
+__global__ void __launch_bounds__(64) k_ion_init(McDev d, int nb, uint32_t first_iter)
 {
     int b = blockIdx.x * blockDim.x + threadIdx.x;
     if (b >= nb) return;
-    unsigned long long c_shi = 0;
     PState I;
     I.E = d.m.ion_E; I.t0 = 0.0; I.X = 0.0; I.Y = 0.0; I.Z = 0.0; I.th = 0.0; I.ph = 0.0;
     I.Ehkin = 0.0; I.Mass = d.m.ion_Zeff0; I.sh = -1;   // Mass holds Zeff for the ion
@@ -1071,94 +1098,226 @@ __global__ void __launch_bounds__(64) k_shi(McDev d, int nb, uint32_t first_iter
     SL = -SL * trk_log(rn(d, I));
     I.tn = t_next(0.0, SL, trk_sqrt(((2.0 * I.E) * kE) / MSHI));
     I.L = SL;
-    while (I.tn < d.T_last) {
-        if (I.ne >= kMaxEvents) { atomicOr(d.overflow, kFlagEvents); break; }
-        uint32_t ev = I.ne++;
-        double t_ev = I.tn;
-        int sh = which_shell(d, d.ES, d.LSsh, d.nS, I.E, I);
-        double dE = shi_transfer(d, sh, I);
-        double SI = nfp_2d(d.ES, d.shi_path, d.nS, I.E);
-        SI = -SI * trk_log(rn(d, I));
-        double Z = I.Z + I.L;
-        I.E = I.E - dE;
-        I.t0 = I.tn;
-        I.Z = Z;
-        I.L = SI;
-        I.tn = t_next(I.t0, SI, trk_sqrt(((2.0 * I.E) * kE) / MSHI));
-        double Zeff = shi_zeff_dev(d, I.E);
-        I.Mass = Zeff;
-        PState ne, nh;
-        init_child(ne, I.id, ev, 0, I.iter, 0);
-        init_child(nh, I.id, ev, 1, I.iter, 1);
-        double dE_cur = receives_E(d, dE, sh, I);
-        double theta = (I.E <= 0.0) ? kPi / 2.0
-                                    : trk_acos(trk_sqrt(((((MSHI + kMe) * (MSHI + kMe)) / ((4.0 * MSHI) * kMe)) * dE) / I.E));
-        double phi = (2.0 * kPi) * rn(d, I);
-        double MFP = el_mfp(d, dE_cur, rn(d, I));
-        double A = 1.0 + MSHI / kMe;
-        double Lb = (((kA0 * Zeff) * kRy) / I.E) * trk_sqrt(((((4.0 * I.E) / dE) * MSHI) / kMe) - A * A);
-        double X = I.X + Lb * trk_sin(phi);
-        double Y = I.Y + Lb * trk_cos(phi);
-        ne.E = dE_cur; ne.t0 = I.t0; ne.X = X; ne.Y = Y; ne.Z = Z; ne.L = MFP; ne.th = theta; ne.ph = phi;
-        ne.tn = t_next(ne.t0, MFP, vel_el(ne.E));
-        cut_el(d, ne);
-        double hth = kPi * rn(d, I);
-        double hph = (2.0 * kPi) * rn(d, I);
-        nh.t0 = I.t0; nh.X = X; nh.Y = Y; nh.Z = Z; nh.sh = sh; nh.th = hth; nh.ph = hph;
-        hole_params(d, nh, dE - dE_cur, I);
-        cut_ho(d, nh);
-        if (Z >= d.m.Layer)
-            I.tn = 1.0e16;
-        emit_pair(d, ne, nh);
-        c_shi++;
-        trace_ev(d, EV_SHI, I, ev, t_ev, I.E, Zeff, I.X, I.Y, I.Z);
-        if (I.E != I.E || I.tn != I.tn) { atomicOr(d.overflow, kFlagNaN); break; }
-    }
-    if (c_shi) atomicAdd(&d.counters[EV_SHI], c_shi);
+    store(d.in[K_ION], b, I, 1);
 }
 
-// Transport of one generation: items [0, np) are electrons, [np, 2np) holes,
-// [2np, 2np+nq) photons.  A lane takes the next item from a shared cursor
-// whenever its particle has finished (below the cut-off, absorbed, or past
-// the last time node), so a wave stays busy until the generation is used up
-// instead of waiting for the longest of its 64 initial histories.  Every lane
-// leaves once the cursor has passed the last item.
-__global__ void __launch_bounds__(256) k_transport(McDev d)
+// One ionising collision of the ion (SHI_Monte_Carlo, MC:2153-2249): shell,
+// energy transfer, next free flight, then the delta electron and the hole.
+TRK_DEV void shi_event(const McDev &d, PState &I)
 {
-    const int np = d.n_in_pairs, nq = d.n_in_phot;
-    const int total = 2 * np + nq;
-    unsigned long long c_ei = 0, c_ee = 0, c_hi = 0, c_he = 0, c_au = 0, c_ra = 0, c_ph = 0;
+    const double MSHI = d.m.ion_mass * kMp;
+    uint32_t ev = I.ne++;
+    double t_ev = I.tn;
+    int sh = which_shell(d, d.ES, d.LSsh, d.nS, I.E, I);
+    double dE = shi_transfer(d, sh, I);
+    double SI = nfp_2d(d.ES, d.shi_path, d.nS, I.E);
+    SI = -SI * trk_log(rn(d, I));
+    double Z = I.Z + I.L;
+    I.E = I.E - dE;
+    I.t0 = I.tn;
+    I.Z = Z;
+    I.L = SI;
+    I.tn = t_next(I.t0, SI, trk_sqrt(((2.0 * I.E) * kE) / MSHI));
+    double Zeff = shi_zeff_dev(d, I.E);
+    I.Mass = Zeff;
+    PState ne, nh;
+    init_child(ne, I.id, ev, 0, I.iter, 0);
+    init_child(nh, I.id, ev, 1, I.iter, 1);
+    double dE_cur = receives_E(d, dE, sh, I);
+    double theta = (I.E <= 0.0) ? kPi / 2.0
+                                : trk_acos(trk_sqrt(((((MSHI + kMe) * (MSHI + kMe)) / ((4.0 * MSHI) * kMe)) * dE) / I.E));
+    double phi = (2.0 * kPi) * rn(d, I);
+    double MFP = el_mfp(d, dE_cur, rn(d, I));
+    double A = 1.0 + MSHI / kMe;
+    double Lb = (((kA0 * Zeff) * kRy) / I.E) * trk_sqrt(((((4.0 * I.E) / dE) * MSHI) / kMe) - A * A);
+    double sphi, cphi;
+    trk_sincos(phi, &sphi, &cphi);
+    double X = I.X + Lb * sphi;
+    double Y = I.Y + Lb * cphi;
+    ne.E = dE_cur; ne.t0 = I.t0; ne.X = X; ne.Y = Y; ne.Z = Z; ne.L = MFP; ne.th = theta; ne.ph = phi;
+    ne.tn = t_next(ne.t0, MFP, vel_el(ne.E));
+    cut_el(d, ne);
+    double hth = kPi * rn(d, I);
+    double hph = (2.0 * kPi) * rn(d, I);
+    nh.t0 = I.t0; nh.X = X; nh.Y = Y; nh.Z = Z; nh.sh = sh; nh.th = hth; nh.ph = hph;
+    hole_params(d, nh, dE - dE_cur, I);
+    cut_ho(d, nh);
+    if (Z >= d.m.Layer)
+        I.tn = 1.0e16;
+    emit_pair(d, ne, nh);
+    trace_ev(d, EV_SHI, I, ev, t_ev, I.E, Zeff, I.X, I.Y, I.Z);
+}
+
+// Category of a live particle's next event, read from the draw that event
+// will use first (the draw is only looked at here; the event consumes it):
+// the electron el/inel test (MC:2301-2309), the VB-hole test (MC:2568-2586),
+// deep-hole decay, photoabsorption and ion collisions.  Fills the free-path
+// cache it needs.
+enum { C_E_EL = 0, C_E_INEL, C_H_EL, C_H_INEL, C_DEEP, C_PHOT, C_ION, C_COUNT };
+
+TRK_DEV int peek_category(const McDev &d, PState &p, int kind)
+{
+    if (kind == K_PH) return C_PHOT;
+    if (kind == K_ION) return C_ION;
+    if (kind == K_HO && p.sh != d.m.vb) return C_DEEP;
+    const double RN = trk_uniform(d.seed, p.id, p.iter, p.nd);
+    if (kind == K_EL) {
+        if (!(p.E == p.cE)) {
+            double lE = trk_log(p.E);
+            int N;
+            p.cIM = nfp_2d_ln(d.E, d.el_imfp, d.lnE, d.ln_el_imfp, d.nE, p.E, lE, &N);
+            p.cEM = nfp_2d_ln(d.Eel, d.Lel, d.lnEel, d.ln_Lel, d.nEel, p.E, lE, &N);
+            p.cN = N;
+            p.cE = p.E;
+        }
+        return (RN * (1.0 / p.cIM + 1.0 / p.cEM) < 1.0 / p.cIM) ? C_E_INEL : C_E_EL;
+    }
+    if (!(p.Ehkin == p.cE)) {
+        double lE = trk_log(p.Ehkin);
+        int N;
+        p.cIM = nfp_2d_ln(d.EH, d.hole_imfp, d.lnEH, d.ln_hole_imfp, d.nH, p.Ehkin, lE, &N);
+        p.cEM = nfp_2d_ln(d.EHel, d.LHel, d.lnEHel, d.ln_LHel, d.nHel, p.Ehkin, lE, &N);
+        p.cN = N;
+        p.cE = p.Ehkin;
+    }
+    return ((RN * (1.0 / p.cIM + 1.0 / p.cEM) < 1.0 / p.cIM) && (p.cIM < 1.0e15)) ? C_H_INEL : C_H_EL;
+}
+
+// A wave runs a category of events only when at least this many of its lanes
+// want it (or when it is the most wanted one); the other lanes wait with
+// their state untouched.  Rare, long branches (ionisation, decays) then run
+// for many lanes at once instead of stalling the wave at almost every step.
+constexpr int kBatchLanes = 16;
+constexpr int kMaxWait = 8;
+
+// Events one particle may take in one round.  A particle with more left is
+// written to the next round's list; this bounds the tail of every launch by
+// the cost of kRoundEvents events instead of the longest history.
+constexpr uint32_t kRoundEvents = 256;
+
+// One round of transport.  Items [0, n_e) are electrons, then holes, photons
+// and ions.  A lane takes the next item from a shared cursor whenever its
+// particle is finished (below the cut-off, absorbed, past the last time
+// node) or has used its event budget for the round, so waves stay full until
+// the round's list is used up; every lane leaves once the cursor has passed
+// the last item.  Children and unfinished particles go to the next round's
+// lists.  A particle's snapshot tallies for the interval up to its next
+// event are added once, when that interval is known (when the particle is
+// loaded and after each event it takes in this round).
+// The small tables every event searches (energy grids, fused free paths and
+// their logs, per-shell free paths, radial and time grids, the VB DOS) are
+// copied into LDS once per workgroup: a bisection step then costs an LDS
+// access instead of an L2 round trip.  Order of the staged arrays:
+//  0-3 E, El_IMFP, ln E, ln El_IMFP      4-7 elastic E, L, ln E, ln L
+//  8-11 hole E, Hole_IMFP, logs          12-15 hole elastic E, L, logs
+//  16 Out_R  17 time nodes  18-21 DOS E, DOS, int_DOS, Eff_m
+//  22 per-shell electron L  23 per-shell hole L
+extern __shared__ double lds_tab[];
+
+TRK_DEV void stage_tables(const McDev &d, McDev &l)
+{
+    const double *src[24] = {d.E, d.el_imfp, d.lnE, d.ln_el_imfp, d.Eel, d.Lel, d.lnEel, d.ln_Lel,
+                             d.EH, d.hole_imfp, d.lnEH, d.ln_hole_imfp, d.EHel, d.LHel, d.lnEHel, d.ln_LHel,
+                             d.R, d.tim_glob, d.m.dosE, d.m.dos, d.m.dosI, d.m.dosM, d.Lsh, d.LHsh};
+#pragma unroll
+    for (int a = 0; a < 24; ++a) {
+        const int n = d.lds_n[a];
+        double *dst = lds_tab + d.lds_off[a];
+        for (int i = threadIdx.x; i < n; i += blockDim.x) dst[i] = src[a][i];
+    }
+    __syncthreads();
+#define TRK_STAGED(a, field) if (d.lds_n[a]) l.field = lds_tab + d.lds_off[a]
+    TRK_STAGED(0, E); TRK_STAGED(1, el_imfp); TRK_STAGED(2, lnE); TRK_STAGED(3, ln_el_imfp);
+    TRK_STAGED(4, Eel); TRK_STAGED(5, Lel); TRK_STAGED(6, lnEel); TRK_STAGED(7, ln_Lel);
+    TRK_STAGED(8, EH); TRK_STAGED(9, hole_imfp); TRK_STAGED(10, lnEH); TRK_STAGED(11, ln_hole_imfp);
+    TRK_STAGED(12, EHel); TRK_STAGED(13, LHel); TRK_STAGED(14, lnEHel); TRK_STAGED(15, ln_LHel);
+    TRK_STAGED(16, R); TRK_STAGED(17, tim_glob); TRK_STAGED(18, m.dosE); TRK_STAGED(19, m.dos);
+    TRK_STAGED(20, m.dosI); TRK_STAGED(21, m.dosM); TRK_STAGED(22, Lsh); TRK_STAGED(23, LHsh);
+#undef TRK_STAGED
+}
+
+constexpr int kTransportBlock = 768;   // 12 waves: one workgroup per CU at 3 waves per SIMD
+
+__global__ void __launch_bounds__(kTransportBlock) k_transport(McDev din)
+{
+    McDev d = din;
+    stage_tables(din, d);
+    // ions first: they start every round at once and feed all other work
+    const int n0 = d.n_in[K_ION], n1 = n0 + d.n_in[K_EL], n2 = n1 + d.n_in[K_HO], total = n2 + d.n_in[K_PH];
+    unsigned long long c_ei = 0, c_ee = 0, c_hi = 0, c_he = 0, c_au = 0, c_ra = 0, c_ph = 0, c_shi = 0;
     PState p;
-    int kind = 0, rep = 0;
+    int kind = 0, rep = 0, cat = 0;
+    uint32_t budget = 0;
+    int waited = 0;
     bool have = false;
     for (;;) {
         if (!have) {
             int t = wave_alloc(d.cursor);
             if (t >= total) break;
-            if (t < np) { kind = 0; load(d.in_e, t, p, 0); }
-            else if (t < 2 * np) { kind = 1; load(d.in_h, t - np, p, 1); }
-            else { kind = 2; load(d.in_p, t - 2 * np, p, 0); }
+            if (t < n0) { kind = K_ION; load(d.in[K_ION], t, p, 1); }
+            else if (t < n1) { kind = K_EL; load(d.in[K_EL], t - n0, p, 0); }
+            else if (t < n2) { kind = K_HO; load(d.in[K_HO], t - n1, p, 1); }
+            else { kind = K_PH; load(d.in[K_PH], t - n2, p, 0); }
             rep = (int)(p.iter % (uint32_t)d.T.R);
-            have = true;
+            budget = kRoundEvents;
+            waited = 0;
+            if (kind != K_ION) segment(d, p, kind, rep);
+            if (p.tn < d.T_last) {
+                if (p.E != p.E) atomicOr(d.overflow, kFlagNaN);
+                else { have = true; cat = peek_category(d, p, kind); }
+            }
         }
-        segment(d, p, kind, rep);
-        if (!(p.tn < d.T_last)) { have = false; continue; }
-        if (p.ne >= kMaxEvents) { atomicOr(d.overflow, kFlagEvents); have = false; continue; }
-        if (p.E != p.E) { atomicOr(d.overflow, kFlagNaN); have = false; continue; }
-        if (kind == 0) {
+        // choose the categories this wave runs now: every category wanted by
+        // kBatchLanes lanes, the most wanted one, the ions (few, and the root
+        // of all other work) and any category a lane has waited kMaxWait
+        // steps for
+        int best = -1, bestn = 0;
+        unsigned run = 1u << C_ION;
+#pragma unroll
+        for (int c = 0; c < C_COUNT; ++c) {
+            int n = __popcll(__ballot(have && cat == c));
+            if (n >= kBatchLanes) run |= 1u << c;
+            if (n > bestn) { bestn = n; best = c; }
+            if (__ballot(have && cat == c && waited >= kMaxWait)) run |= 1u << c;
+        }
+        if (best >= 0) run |= 1u << best;
+        if (!have) continue;
+        if (!((run >> cat) & 1u)) { ++waited; continue; }
+        waited = 0;
+
+        if (kind == K_EL) {
             int k = electron_event(d, p, rep);
             if (k == EV_E_INEL) c_ei++; else c_ee++;
-        } else if (kind == 1) {
+        } else if (kind == K_HO) {
             int k = hole_event(d, p, rep);
             if (k == EV_H_INEL) c_hi++;
             else if (k == EV_H_EL) c_he++;
             else if (k == EV_AUGER) c_au++;
             else c_ra++;
-        } else {
+        } else if (kind == K_PH) {
             photon_event(d, p);
             c_ph++;
             have = false;
+            continue;
+        } else {
+            shi_event(d, p);
+            c_shi++;
         }
+        if (p.ne >= kMaxEvents) { atomicOr(d.overflow, kFlagEvents); have = false; continue; }
+        if (p.E != p.E || p.tn != p.tn) { atomicOr(d.overflow, kFlagNaN); have = false; continue; }
+        if (--budget == 0 && p.tn < d.T_last) {
+            // continues next round; its next interval is tallied then.  The
+            // append is made per kind so that every lane in one wave-wide
+            // allocation uses the same list counter.
+#pragma unroll
+            for (int k = 0; k < 4; ++k)
+                if (kind == k) emit(d, k, p);
+            have = false;
+            continue;
+        }
+        if (kind != K_ION) segment(d, p, kind, rep);
+        if (!(p.tn < d.T_last)) { have = false; continue; }
+        cat = peek_category(d, p, kind);
     }
     if (c_ei) atomicAdd(&d.counters[EV_E_INEL], c_ei);
     if (c_ee) atomicAdd(&d.counters[EV_E_EL], c_ee);
@@ -1167,14 +1326,17 @@ __global__ void __launch_bounds__(256) k_transport(McDev d)
     if (c_au) atomicAdd(&d.counters[EV_AUGER], c_au);
     if (c_ra) atomicAdd(&d.counters[EV_RAD], c_ra);
     if (c_ph) atomicAdd(&d.counters[EV_PHOT], c_ph);
+    if (c_shi) atomicAdd(&d.counters[EV_SHI], c_shi);
 }
 
 // ---------------------------------------------------------------- host side
 struct McState {
-    PList L[2][3];            // [ping-pong][e, h, p]
-    int cap_pairs = 0, cap_phot = 0;
-    int *d_counts = nullptr;  // [8]: pairs A, phot A, pairs B, phot B, transport cursor
+    PList L[2][4];            // [ping-pong][electrons, holes, photons, ions]
+    int cap[4] = {0, 0, 0, 0};
+    int *d_counts = nullptr;  // [9]: list counts of round A [4], of round B [4], transport cursor
     int64_t max_blocks = 0;   // resident transport workgroups on the device
+    int ncu = 0, max_lds = 0;
+    size_t lds_bytes = 0;     // staged tables per transport workgroup
     int *d_overflow = nullptr;
     unsigned long long *d_counters = nullptr;
     Tally T{};
@@ -1296,27 +1458,26 @@ int run_mc(trk_ctx *c, int64_t nmc, uint64_t seed, int64_t first_iter, const dou
     McState *s = c->mc;
     const int Nt = h.Nt, nsh = h.nsh;
     const int R = (c->hm.o.tally_replicas > 0) ? c->hm.o.tally_replicas : 8;
-    // capacities: pairs per generation
+    // capacities: particles of each kind per round
     int cap = (c->hm.o.gen_capacity > 0) ? c->hm.o.gen_capacity : (1 << 22);
-    if (s->cap_pairs != cap) {
-        for (auto &a : s->L) for (auto &l : a) free_list(l);
-        for (int g = 0; g < 2; ++g) {
-            if (alloc_list(c, s->L[g][0], cap, 0)) return TRK_ERR_DEVICE;
-            if (alloc_list(c, s->L[g][1], cap, 1)) return TRK_ERR_DEVICE;
-            if (alloc_list(c, s->L[g][2], cap / 4, 0)) return TRK_ERR_DEVICE;
-        }
-        s->cap_pairs = cap;
-        s->cap_phot = cap / 4;
-        if (!s->d_counts) HIPC(hipMalloc((void **)&s->d_counts, 8 * sizeof(int)));
+    if (s->cap[0] != cap) {
+        for (int g = 0; g < 2; ++g)
+            for (int k = 0; k < 3; ++k) free_list(s->L[g][k]);
+        const int caps[3] = {cap, cap, cap / 4};
+        for (int g = 0; g < 2; ++g)
+            for (int k = 0; k < 3; ++k)
+                if (alloc_list(c, s->L[g][k], caps[k], k == K_HO)) return TRK_ERR_DEVICE;
+        for (int k = 0; k < 3; ++k) s->cap[k] = caps[k];
+        if (!s->d_counts) HIPC(hipMalloc((void **)&s->d_counts, 16 * sizeof(int)));
         if (!s->d_overflow) HIPC(hipMalloc((void **)&s->d_overflow, sizeof(int)));
         if (!s->d_counters) HIPC(hipMalloc((void **)&s->d_counters, 16 * sizeof(unsigned long long)));
         if (!s->ev0) { HIPC(hipEventCreate(&s->ev0)); HIPC(hipEventCreate(&s->ev1)); }
         int per_cu = 0, dev = 0, ncu = 0;
         HIPC(hipGetDevice(&dev));
         HIPC(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
-        HIPC(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_transport, 256, 0));
-        if (per_cu < 1) per_cu = 1;
-        s->max_blocks = (int64_t)per_cu * ncu;
+        s->ncu = ncu;
+        HIPC(hipDeviceGetAttribute(&s->max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
+        (void)per_cu;
     }
     if (ensure_tally(c, s, Nt, NR, nsh, R)) return TRK_ERR_DEVICE;
     if (s->d_tim) (void)hipFree(s->d_tim);
@@ -1366,7 +1527,6 @@ int run_mc(trk_ctx *c, int64_t nmc, uint64_t seed, int64_t first_iter, const dou
     }
     d.NR = NR; d.R = s->d_R;
     d.include_photons = h.o.include_photons;
-    d.cap_pairs = s->cap_pairs; d.cap_phot = s->cap_phot;
     d.overflow = s->d_overflow;
     d.counters = s->d_counters;
     d.T = T;
@@ -1394,25 +1554,63 @@ int run_mc(trk_ctx *c, int64_t nmc, uint64_t seed, int64_t first_iter, const dou
     int64_t B = (c->hm.o.batch > 0) ? c->hm.o.batch : (int64_t)((double)cap / (3.0 * est_pairs + 64.0));
     if (B < 1) B = 1;
     if (B > nmc) B = nmc;
+    if (s->cap[K_ION] < B) {
+        for (int g = 0; g < 2; ++g) {
+            free_list(s->L[g][K_ION]);
+            if (alloc_list(c, s->L[g][K_ION], (int)B, 1)) return TRK_ERR_DEVICE;
+        }
+        s->cap[K_ION] = (int)B;
+    }
+    for (int k = 0; k < 4; ++k) d.cap[k] = s->cap[k];
+
+    // LDS plan: stage the arrays in order while they fit
+    {
+        const int nsz[24] = {d.nE, d.nE, d.nE, d.nE, d.nEel, d.nEel, d.nEel, d.nEel,
+                             d.nH, d.nH, d.nH, d.nH, d.nHel, d.nHel, d.nHel, d.nHel,
+                             NR, Nt, h.m.n_dos, h.m.n_dos, h.m.n_dos, h.m.n_dos, nsh * d.nE, nsh * d.nH};
+        const bool present[24] = {true, true, d.lnE != nullptr, d.ln_el_imfp != nullptr, true, true,
+                                  d.lnEel != nullptr, d.ln_Lel != nullptr, true, true, d.lnEH != nullptr,
+                                  d.ln_hole_imfp != nullptr, true, true, d.lnEHel != nullptr, d.ln_LHel != nullptr,
+                                  true, true, true, true, true, true, true, true};
+        const size_t limit = (size_t)s->max_lds / sizeof(double);
+        size_t off = 0;
+        for (int a = 0; a < 24; ++a) {
+            size_t n = present[a] ? (size_t)nsz[a] : 0;
+            if (off + n > limit) n = 0;
+            d.lds_n[a] = (int)n;
+            d.lds_off[a] = (int)off;
+            off += n;
+        }
+        s->lds_bytes = off * sizeof(double);
+        if (s->lds_bytes > 65536)
+            HIPC(hipFuncSetAttribute((const void *)k_transport, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                     (int)s->lds_bytes));
+        int per_cu = 0;
+        HIPC(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_transport, kTransportBlock, s->lds_bytes));
+        if (per_cu < 1) per_cu = 1;
+        s->max_blocks = (int64_t)per_cu * s->ncu;
+    }
 
     double transport_s = 0.0;
     int64_t launches = 0;
     std::vector<int> hc(8);
+    const bool verbose = getenv("TRK_VERBOSE") != nullptr;   // per-round list sizes on stderr
     for (int64_t b0 = 0; b0 < nmc; b0 += B) {
         int nb = (int)((nmc - b0 < B) ? nmc - b0 : B);
-        int cur = 0, gens = 0;
-        HIPC(hipMemsetAsync(s->d_counts, 0, 8 * sizeof(int), st));
-        d.out_e = s->L[cur][0]; d.out_h = s->L[cur][1]; d.out_p = s->L[cur][2];
-        d.out_pairs = s->d_counts + 0; d.out_phot = s->d_counts + 1;
+        int cur = 0, rounds = 0;
+        for (int k = 0; k < 4; ++k) d.in[k] = s->L[cur][k];
         HIPC(hipEventRecord(s->ev0, st));
         if (c->prof) HIPC(hipEventRecord(c->pe[0], st));
-        k_shi<<<(nb + 63) / 64, 64, 0, st>>>(d, nb, (uint32_t)(first_iter + b0));
+        k_ion_init<<<(nb + 63) / 64, 64, 0, st>>>(d, nb, (uint32_t)(first_iter + b0));
         HIPC(hipGetLastError());
         if (c->prof) HIPC(hipEventRecord(c->pe[1], st));
-        int pending = 0;          // kernel whose events are still to be read
         launches++;
+        int pending = 0;          // kernel whose events are still to be read
+        int nin[4] = {0, 0, 0, nb};
         for (;;) {
-            HIPC(hipMemcpyAsync(hc.data(), s->d_counts, 8 * sizeof(int), hipMemcpyDeviceToHost, st));
+            if (rounds > 0) {
+                HIPC(hipMemcpyAsync(hc.data(), s->d_counts + 4 * cur, 4 * sizeof(int), hipMemcpyDeviceToHost, st));
+            }
             HIPC(hipStreamSynchronize(st));
             if (c->prof && pending >= 0) {
                 float kms = 0.f;
@@ -1421,30 +1619,33 @@ int run_mc(trk_ctx *c, int64_t nmc, uint64_t seed, int64_t first_iter, const dou
                 c->k_launches[pending]++;
                 pending = -1;
             }
-            int np = hc[2 * cur + 0], nq = hc[2 * cur + 1];
             int ovf = 0;
             HIPC(hipMemcpy(&ovf, s->d_overflow, sizeof(int), hipMemcpyDeviceToHost));
             if (ovf & kFlagNaN) return set_error(c, TRK_ERR_NUMERIC, "a particle state became NaN (iterations %lld..%lld)",
                                                   (long long)(first_iter + b0), (long long)(first_iter + b0 + nb - 1));
-            if (ovf & kFlagEvents) return set_error(c, TRK_ERR_CAPACITY, "a particle exceeded %u events in one launch", kMaxEvents);
-            if (ovf) return set_error(c, TRK_ERR_CAPACITY, "particle list capacity %d exceeded (batch %lld)", cap, (long long)B);
-            if (++gens > kMaxGenerations) return set_error(c, TRK_ERR_CAPACITY, "more than %d particle generations", kMaxGenerations);
-            if (np > s->cap_pairs) np = s->cap_pairs;
-            if (nq > s->cap_phot) nq = s->cap_phot;
-            if (np == 0 && nq == 0) break;
+            if (ovf & kFlagEvents) return set_error(c, TRK_ERR_CAPACITY, "a particle exceeded %u events", kMaxEvents);
+            if (ovf)
+                return set_error(c, TRK_ERR_CAPACITY,
+                                 "particle list capacity exceeded (flags %d, round %d, batch %lld; capacities e %d h %d p %d ion %d)",
+                                 ovf, rounds, (long long)B, s->cap[0], s->cap[1], s->cap[2], s->cap[3]);
+            if (++rounds > kMaxGenerations) return set_error(c, TRK_ERR_CAPACITY, "more than %d transport rounds", kMaxGenerations);
+            if (rounds > 1)
+                for (int k = 0; k < 4; ++k) nin[k] = hc[k] < s->cap[k] ? hc[k] : s->cap[k];
+            if (verbose)
+                fprintf(stderr, "[trk] batch %lld round %d: electrons %d holes %d photons %d ions %d\n",
+                        (long long)b0, rounds, nin[0], nin[1], nin[2], nin[3]);
+            int64_t nthr = (int64_t)nin[0] + nin[1] + nin[2] + nin[3];
+            if (nthr == 0) break;
             int nxt = 1 - cur;
-            HIPC(hipMemsetAsync(s->d_counts + 2 * nxt, 0, 2 * sizeof(int), st));
-            d.in_e = s->L[cur][0]; d.in_h = s->L[cur][1]; d.in_p = s->L[cur][2];
-            d.n_in_pairs = np; d.n_in_phot = nq;
-            d.out_e = s->L[nxt][0]; d.out_h = s->L[nxt][1]; d.out_p = s->L[nxt][2];
-            d.out_pairs = s->d_counts + 2 * nxt; d.out_phot = s->d_counts + 2 * nxt + 1;
-            int64_t nthr = 2 * (int64_t)np + nq;
-            int64_t nblk = (nthr + 255) / 256;
+            HIPC(hipMemsetAsync(s->d_counts + 4 * nxt, 0, 4 * sizeof(int), st));
+            for (int k = 0; k < 4; ++k) { d.in[k] = s->L[cur][k]; d.out[k] = s->L[nxt][k]; d.n_in[k] = nin[k]; }
+            d.out_cnt = s->d_counts + 4 * nxt;
+            int64_t nblk = (nthr + kTransportBlock - 1) / kTransportBlock;
             if (nblk > s->max_blocks) nblk = s->max_blocks;
-            d.cursor = s->d_counts + 4;
+            d.cursor = s->d_counts + 8;
             HIPC(hipMemsetAsync(d.cursor, 0, sizeof(int), st));
             if (c->prof) HIPC(hipEventRecord(c->pe[0], st));
-            k_transport<<<(unsigned)nblk, 256, 0, st>>>(d);
+            k_transport<<<(unsigned)nblk, kTransportBlock, s->lds_bytes, st>>>(d);
             HIPC(hipGetLastError());
             if (c->prof) { HIPC(hipEventRecord(c->pe[1], st)); pending = 1; }
             launches++;
