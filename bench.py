@@ -56,10 +56,11 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--iters-per-step", type=int, default=64)
-    ap.add_argument("--cpu-iters", type=int, default=1, help="oracle iterations for the CPU baseline (0 = skip)")
+    ap.add_argument("--iters-per-step", type=int, default=4096)
+    ap.add_argument("--cpu-iters", type=int, default=2, help="oracle iterations for the CPU baseline (0 = skip)")
     ap.add_argument("--config", default="config2")
-    ap.add_argument("--gen-capacity", type=int, default=1 << 24, help="particle pairs per generation buffer")
+    ap.add_argument("--gen-capacity", type=int, default=1 << 27,
+                    help="particles of one kind per round list (electrons, holes; photons get a quarter)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run a plain device-to-device copy of MIB MiB five times (the reference kernel "
                          "that scales HBM byte counters) and exit")

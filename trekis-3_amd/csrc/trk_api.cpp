@@ -265,6 +265,8 @@ int trk_run_mc(trk_ctx *c, int64_t nmc, uint64_t seed, int64_t first_iter, const
         return set_error(c, TRK_ERR_ARG, "invalid trk_run_mc arguments");
     if (out->Nt != c->hm.Nt) return set_error(c, TRK_ERR_ARG, "tallies have Nt=%d, the run has %d time nodes", out->Nt, c->hm.Nt);
     if (out->NR != NR) return set_error(c, TRK_ERR_ARG, "tallies have NR=%d, the radial grid has %d bins", out->NR, NR);
+    if (out->Out_Eh_vs_E && out->n_dos != c->hm.m.n_dos)
+        return set_error(c, TRK_ERR_ARG, "tallies have n_dos=%d, the material's DOS has %d points", out->n_dos, c->hm.m.n_dos);
     for (int j = 1; j < NR; ++j)
         if (!(Out_R[j] > Out_R[j - 1])) return set_error(c, TRK_ERR_ARG, "Out_R must be strictly increasing");
     return run_mc(c, nmc, seed, first_iter, Out_R, Out_V, NR, out);

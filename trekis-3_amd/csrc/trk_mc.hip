@@ -53,6 +53,19 @@ struct Tally {            // fixed-point difference arrays, replicated R times
     unsigned long long *Elat;                      // [R][Nt][NR][2] (not differenced)
 };
 
+// Per-iteration snapshot histograms for the tallies the reference normalises
+// inside each iteration (Out_Ee_vs_E, Out_Eh_vs_E, Out_theta, Out_theta_h by
+// that iteration's particle counts, Out_diff_coeff by its VB-hole count),
+// difference arrays over time like the Tally arrays.  b = iteration - iter0.
+struct Spec {
+    int on, B, ND, iter0;
+    int *nel, *nvb, *dcn;          // [B][Nt]
+    int *eve;                      // [B][Nt][NR]   electron spectrum, bins of Out_R
+    int *ehve;                     // [B][Nt][ND]   VB-hole spectrum, bins of the DOS grid
+    int *th, *thh;                 // [B][Nt][180]  polar angle of electrons / moving holes
+    unsigned long long *dcs;       // [B][Nt][2]    sum of v*L/3 of mobile holes, fixed point
+};
+
 struct McDev {
     MatDev m;
     // tables
@@ -85,6 +98,7 @@ struct McDev {
     unsigned long long *counters;       // [16] event counters
     double *trace; unsigned long long *trace_n; long long trace_cap;
     Tally T;
+    Spec S;
 };
 
 enum { EV_SHI = 0, EV_E_INEL, EV_E_EL, EV_H_INEL, EV_H_EL, EV_AUGER, EV_RAD, EV_PHOT };
@@ -214,11 +228,62 @@ TRK_DEV void range_h(const McDev &d, int rep, int r0, int r1, int j, int s, doub
 // Snapshot contribution of one state, valid from its creation time t0 until
 // its next event tn (MC:998-1097): nodes i with t0 < tim_glob(i) <= tn.
 // kind 0 electron, 1 hole, 2 photon.
+TRK_DEV double vel_hole(const PState &h);
+
+TRK_DEV void range_i(int *a, int stride, int Nt, int r0, int r1)
+{
+    atomicAdd(a + (size_t)r0 * stride, 1);
+    if (r1 + 1 < Nt) atomicAdd(a + (size_t)(r1 + 1) * stride, -1);
+}
+
+// bin of a polar angle on the 1-degree grid 1..180 (find_mono_1d on it)
+TRK_DEV int theta_bin(double th)
+{
+    double xx = (th / kPi) * 180.0;
+    if (!(xx >= 1.0)) return 0;
+    if (xx >= 180.0) return 179;
+    int b = (int)xx;              // number of grid values <= xx
+    return b < 179 ? b : 179;
+}
+
+// Calculated_statistics, MC:1022-1110: the per-iteration normalised tallies
+TRK_DEV void spectra(const McDev &d, const PState &p, int kind, int r0, int r1)
+{
+    const Spec &S = d.S;
+    const int Nt = d.Nt;
+    const size_t b = (size_t)(p.iter - (uint32_t)S.iter0);
+    if (kind == 0) {
+        range_i(S.nel + b * Nt, 1, Nt, r0, r1);
+        int j = find_mono_1d(d.R, d.NR, p.E) - 1;
+        range_i(S.eve + (b * Nt) * d.NR + j, d.NR, Nt, r0, r1);
+        if (p.E > 0.0) {
+            double th0 = (p.E > d.cutoff_eff) ? p.th : 0.0;
+            range_i(S.th + (b * Nt) * 180 + theta_bin(th0), 180, Nt, r0, r1);
+        }
+    } else if (kind == 1) {
+        if (p.sh == d.m.vb) {
+            range_i(S.nvb + b * Nt, 1, Nt, r0, r1);
+            if (S.ND > 1) {
+                int jj = find_mono_1d(d.m.dosE, d.m.n_dos, p.Ehkin) - 1;
+                range_i(S.ehve + (b * Nt) * S.ND + jj, S.ND, Nt, r0, r1);
+            }
+        }
+        if (p.Mass < 1.0e3 && p.Ehkin > d.cutoff_eff) range_i(S.thh + (b * Nt) * 180 + theta_bin(p.th), 180, Nt, r0, r1);
+        if (p.Mass < 1.0e3 && p.L < 1.0e3) {
+            range_i(S.dcn + b * Nt, 1, Nt, r0, r1);
+            double v = (((1.0 / 3.0) * vel_hole(p)) * p.L) * 1.0e-6;
+            fx_add(S.dcs + 2 * (b * Nt + r0), v);
+            if (r1 + 1 < Nt) fx_add(S.dcs + 2 * (b * Nt + r1 + 1), -v);
+        }
+    }
+}
+
 TRK_DEV void segment(const McDev &d, const PState &p, int kind, int rep)
 {
     int r0 = nodes_le(d, p.t0);                         // first node row with tim > t0
     int r1 = (p.tn < d.T_last) ? nodes_le(d, p.tn) - 1 : d.Nt - 1;
     if (r0 > r1) return;
+    if (d.S.on) spectra(d, p, kind, r0, r1);
     double V = 0.0, st = 0.0, sp = 0.0, cp = 0.0;
     bool moving;
     if (kind == 0) {
@@ -1329,6 +1394,71 @@ __global__ void __launch_bounds__(kTransportBlock) k_transport(McDev din)
     if (c_shi) atomicAdd(&d.counters[EV_SHI], c_shi);
 }
 
+// ---------------------------------------------------------------- per-iteration tallies
+// Prefix sums over time of [B][Nt][ncol] difference arrays, one thread per column.
+__global__ void k_prefix_i(int *a, int B, int Nt, int ncol)
+{
+    int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= (int64_t)B * ncol) return;
+    int64_t b = c / ncol, col = c % ncol;
+    int *p = a + (size_t)b * Nt * ncol + col;
+    int run = 0;
+    for (int i = 0; i < Nt; ++i) { run += p[(size_t)i * ncol]; p[(size_t)i * ncol] = run; }
+}
+
+// Same for the fixed-point sums [B][Nt][2]; the running value is left in word 0.
+__global__ void k_prefix_fx(unsigned long long *a, int B, int Nt)
+{
+    int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= B) return;
+    unsigned long long *p = a + (size_t)b * Nt * 2;
+    long long run = 0;
+    for (int i = 0; i < Nt; ++i) {
+        run += (long long)p[2 * i + 1] * 4294967296ll + (long long)p[2 * i];
+        p[2 * i] = (unsigned long long)run;
+        p[2 * i + 1] = 0ull;
+    }
+}
+
+// acc(i, j) += sum over iterations of count * (w_j / norm), in iteration order.
+// wmode 0: w = 1 (angle histograms); 1: w from the radial grid used as an
+// energy grid (MC:1024-1029); 2: w from the DOS grid (MC:1033-1040).
+__global__ void k_norm_hist(double *acc, int accld, const int *cnt, const int *norm, int B, int Nt, int ncol, int wmode,
+                            const double *grid)
+{
+    int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (int64_t)Nt * ncol) return;
+    int i = (int)(t % Nt), j = (int)(t / Nt);
+    double w = 1.0;
+    if (wmode == 1) w = (j > 0) ? 1.0 / (grid[j] - grid[j - 1]) : 1.0 / grid[0];
+    else if (wmode == 2) w = (j > 0) ? 1.0 / (grid[j] - grid[j - 1]) : 1.0 / (grid[1] - grid[0]);
+    double sum = acc[(size_t)i + (size_t)accld * j];
+    for (int b = 0; b < B; ++b) {
+        int c = cnt[((size_t)b * Nt + i) * ncol + j];
+        if (c) sum = sum + (double)c * (w / (double)norm[(size_t)b * Nt + i]);
+    }
+    acc[(size_t)i + (size_t)accld * j] = sum;
+}
+
+// Out_diff_coeff recurrence of MC:1094-1097: every iteration divides the
+// running value plus its own sum by its own count.
+__global__ void k_diff_coeff(double *D, const unsigned long long *dcs, const int *dcn, int B, int Nt)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= Nt) return;
+    double v = D[i];
+    for (int b = 0; b < B; ++b) {
+        int n = dcn[(size_t)b * Nt + i];
+        if (n > 0) {
+            long long x = (long long)dcs[((size_t)b * Nt + i) * 2];
+            long long hi = x >> 40;
+            double s = (double)hi + (double)(x - (hi << 40)) * 0x1.0p-40;
+            v = (v + s) / (double)n;
+        }
+    }
+    D[i] = v;
+}
+
 // ---------------------------------------------------------------- host side
 struct McState {
     PList L[2][4];            // [ping-pong][electrons, holes, photons, ions]
@@ -1347,6 +1477,10 @@ struct McState {
     unsigned long long *d_trace_n = nullptr;
     int64_t trace_cap = 0;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    // per-iteration tallies (allocated for spec_B iterations) and their accumulators
+    Spec S{};
+    int spec_B = 0, spec_Nt = 0, spec_NR = 0, spec_ND = 0;
+    double *acc_eve = nullptr, *acc_ehve = nullptr, *acc_th = nullptr, *acc_thh = nullptr, *acc_dc = nullptr;
 };
 
 #define HIPC(x)                                                                          \
@@ -1382,9 +1516,20 @@ static void free_list(PList &l)
     l = PList{};
 }
 
+static void free_spec(McState *s)
+{
+    void *ps[] = {s->S.nel, s->S.nvb, s->S.dcn, s->S.eve, s->S.ehve, s->S.th, s->S.thh, s->S.dcs,
+                  s->acc_eve, s->acc_ehve, s->acc_th, s->acc_thh, s->acc_dc};
+    for (void *p : ps) if (p) (void)hipFree(p);
+    s->S = Spec{};
+    s->acc_eve = s->acc_ehve = s->acc_th = s->acc_thh = s->acc_dc = nullptr;
+    s->spec_B = 0;
+}
+
 void mc_free(McState *s)
 {
     if (!s) return;
+    free_spec(s);
     for (auto &a : s->L) for (auto &l : a) free_list(l);
     void *ps[] = {s->d_counts, s->d_overflow, s->d_counters, s->T.cnt_e, s->T.E_e, s->T.cnt_h, s->T.E_h,
                   s->T.Ek_h, s->T.cnt_p, s->T.E_p, s->T.Elat, s->d_tim, s->d_R, s->d_trace, s->d_trace_n};
@@ -1412,6 +1557,29 @@ static int ensure_tally(trk_ctx *c, McState *s, int Nt, int NR, int nsh, int R)
     HIPC(hipMalloc((void **)&T.Elat, cell * 16));
     s->tally_bytes = cell * (8 + 16 + nsh * 40 + 8 + 16 + 16);
     s->Nt_alloc = Nt; s->NR_alloc = NR; s->R_alloc = R;
+    return 0;
+}
+
+static int ensure_spec(trk_ctx *c, McState *s, int B, int Nt, int NR, int ND)
+{
+    if (s->spec_B >= B && s->spec_Nt == Nt && s->spec_NR == NR && s->spec_ND == ND) return 0;
+    free_spec(s);
+    Spec &S = s->S;
+    const size_t bt = (size_t)B * Nt;
+    HIPC(hipMalloc((void **)&S.nel, bt * sizeof(int)));
+    HIPC(hipMalloc((void **)&S.nvb, bt * sizeof(int)));
+    HIPC(hipMalloc((void **)&S.dcn, bt * sizeof(int)));
+    HIPC(hipMalloc((void **)&S.eve, bt * NR * sizeof(int)));
+    HIPC(hipMalloc((void **)&S.ehve, bt * (ND > 1 ? ND : 1) * sizeof(int)));
+    HIPC(hipMalloc((void **)&S.th, bt * 180 * sizeof(int)));
+    HIPC(hipMalloc((void **)&S.thh, bt * 180 * sizeof(int)));
+    HIPC(hipMalloc((void **)&S.dcs, bt * 2 * sizeof(unsigned long long)));
+    HIPC(hipMalloc((void **)&s->acc_eve, (size_t)Nt * NR * sizeof(double)));
+    HIPC(hipMalloc((void **)&s->acc_ehve, (size_t)Nt * (ND > 1 ? ND : 1) * sizeof(double)));
+    HIPC(hipMalloc((void **)&s->acc_th, (size_t)(Nt + 1) * 180 * sizeof(double)));
+    HIPC(hipMalloc((void **)&s->acc_thh, (size_t)(Nt + 1) * 180 * sizeof(double)));
+    HIPC(hipMalloc((void **)&s->acc_dc, (size_t)Nt * sizeof(double)));
+    s->spec_B = B; s->spec_Nt = Nt; s->spec_NR = NR; s->spec_ND = ND;
     return 0;
 }
 
@@ -1563,6 +1731,25 @@ int run_mc(trk_ctx *c, int64_t nmc, uint64_t seed, int64_t first_iter, const dou
     }
     for (int k = 0; k < 4; ++k) d.cap[k] = s->cap[k];
 
+    // per-iteration normalised tallies, when the caller asks for any of them
+    const bool want_spec = out->Out_Ee_vs_E || out->Out_Eh_vs_E || out->Out_theta || out->Out_theta_h || out->Out_diff_coeff;
+    const int ND = h.m.n_dos;
+    d.S = Spec{};
+    if (want_spec) {
+        if (ensure_spec(c, s, (int)B, Nt, NR, ND)) return TRK_ERR_DEVICE;
+        HIPC(hipMemsetAsync(s->acc_eve, 0, (size_t)Nt * NR * sizeof(double), st));
+        HIPC(hipMemsetAsync(s->acc_ehve, 0, (size_t)Nt * (ND > 1 ? ND : 1) * sizeof(double), st));
+        HIPC(hipMemsetAsync(s->acc_th, 0, (size_t)(Nt + 1) * 180 * sizeof(double), st));
+        HIPC(hipMemsetAsync(s->acc_thh, 0, (size_t)(Nt + 1) * 180 * sizeof(double), st));
+        if (out->Out_diff_coeff)
+            HIPC(hipMemcpyAsync(s->acc_dc, out->Out_diff_coeff, (size_t)Nt * sizeof(double), hipMemcpyHostToDevice, st));
+        else
+            HIPC(hipMemsetAsync(s->acc_dc, 0, (size_t)Nt * sizeof(double), st));
+        d.S = s->S;
+        d.S.on = 1;
+        d.S.ND = ND;
+    }
+
     // LDS plan: stage the arrays in order while they fit
     {
         const int nsz[24] = {d.nE, d.nE, d.nE, d.nE, d.nEel, d.nEel, d.nEel, d.nEel,
@@ -1598,6 +1785,19 @@ int run_mc(trk_ctx *c, int64_t nmc, uint64_t seed, int64_t first_iter, const dou
     for (int64_t b0 = 0; b0 < nmc; b0 += B) {
         int nb = (int)((nmc - b0 < B) ? nmc - b0 : B);
         int cur = 0, rounds = 0;
+        if (want_spec) {
+            const size_t bt = (size_t)nb * Nt;
+            HIPC(hipMemsetAsync(s->S.nel, 0, bt * sizeof(int), st));
+            HIPC(hipMemsetAsync(s->S.nvb, 0, bt * sizeof(int), st));
+            HIPC(hipMemsetAsync(s->S.dcn, 0, bt * sizeof(int), st));
+            HIPC(hipMemsetAsync(s->S.eve, 0, bt * NR * sizeof(int), st));
+            HIPC(hipMemsetAsync(s->S.ehve, 0, bt * (ND > 1 ? ND : 1) * sizeof(int), st));
+            HIPC(hipMemsetAsync(s->S.th, 0, bt * 180 * sizeof(int), st));
+            HIPC(hipMemsetAsync(s->S.thh, 0, bt * 180 * sizeof(int), st));
+            HIPC(hipMemsetAsync(s->S.dcs, 0, bt * 2 * sizeof(unsigned long long), st));
+            d.S.B = nb;
+            d.S.iter0 = (int)(first_iter + b0);
+        }
         for (int k = 0; k < 4; ++k) d.in[k] = s->L[cur][k];
         HIPC(hipEventRecord(s->ev0, st));
         if (c->prof) HIPC(hipEventRecord(c->pe[0], st));
@@ -1650,6 +1850,25 @@ int run_mc(trk_ctx *c, int64_t nmc, uint64_t seed, int64_t first_iter, const dou
             if (c->prof) { HIPC(hipEventRecord(c->pe[1], st)); pending = 1; }
             launches++;
             cur = nxt;
+        }
+        if (want_spec) {
+            const Spec &S = s->S;
+            auto blk = [](int64_t n) { return (unsigned)((n + 255) / 256); };
+            k_prefix_i<<<blk(nb), 256, 0, st>>>(S.nel, nb, Nt, 1);
+            k_prefix_i<<<blk(nb), 256, 0, st>>>(S.nvb, nb, Nt, 1);
+            k_prefix_i<<<blk(nb), 256, 0, st>>>(S.dcn, nb, Nt, 1);
+            k_prefix_i<<<blk((int64_t)nb * NR), 256, 0, st>>>(S.eve, nb, Nt, NR);
+            if (ND > 1) k_prefix_i<<<blk((int64_t)nb * ND), 256, 0, st>>>(S.ehve, nb, Nt, ND);
+            k_prefix_i<<<blk((int64_t)nb * 180), 256, 0, st>>>(S.th, nb, Nt, 180);
+            k_prefix_i<<<blk((int64_t)nb * 180), 256, 0, st>>>(S.thh, nb, Nt, 180);
+            k_prefix_fx<<<blk(nb), 256, 0, st>>>(S.dcs, nb, Nt);
+            k_norm_hist<<<blk((int64_t)Nt * NR), 256, 0, st>>>(s->acc_eve, Nt, S.eve, S.nel, nb, Nt, NR, 1, s->d_R);
+            if (ND > 1)
+                k_norm_hist<<<blk((int64_t)Nt * ND), 256, 0, st>>>(s->acc_ehve, Nt, S.ehve, S.nvb, nb, Nt, ND, 2, c->md.dosE);
+            k_norm_hist<<<blk((int64_t)Nt * 180), 256, 0, st>>>(s->acc_th, Nt + 1, S.th, S.nel, nb, Nt, 180, 0, nullptr);
+            k_norm_hist<<<blk((int64_t)Nt * 180), 256, 0, st>>>(s->acc_thh, Nt + 1, S.thh, S.nvb, nb, Nt, 180, 0, nullptr);
+            k_diff_coeff<<<blk(Nt), 256, 0, st>>>(s->acc_dc, S.dcs, S.dcn, nb, Nt);
+            HIPC(hipGetLastError());
         }
         HIPC(hipEventRecord(s->ev1, st));
         HIPC(hipEventSynchronize(s->ev1));
@@ -1747,6 +1966,22 @@ int run_mc(trk_ctx *c, int64_t nmc, uint64_t seed, int64_t first_iter, const dou
             if (out->Out_E_phot) out->Out_E_phot[i] += fx_to_double(tot_Ep);
         }
         if (out->Out_tot_E) out->Out_tot_E[i] += fx_to_double(tot_Ee + hsum + (h.o.include_photons ? tot_Ep : 0) + at_cum);
+    }
+    if (want_spec) {
+        std::vector<double> buf;
+        auto add_back = [&](double *dst, const double *src, size_t n) -> int {
+            if (!dst) return 0;
+            buf.resize(n);
+            HIPC(hipMemcpy(buf.data(), src, n * sizeof(double), hipMemcpyDeviceToHost));
+            for (size_t k = 0; k < n; ++k) dst[k] += buf[k];
+            return 0;
+        };
+        if (add_back(out->Out_Ee_vs_E, s->acc_eve, (size_t)Nt * NR)) return TRK_ERR_DEVICE;
+        if (ND > 1 && add_back(out->Out_Eh_vs_E, s->acc_ehve, (size_t)Nt * ND)) return TRK_ERR_DEVICE;
+        if (add_back(out->Out_theta, s->acc_th, (size_t)(Nt + 1) * 180)) return TRK_ERR_DEVICE;
+        if (add_back(out->Out_theta_h, s->acc_thh, (size_t)(Nt + 1) * 180)) return TRK_ERR_DEVICE;
+        if (out->Out_diff_coeff)
+            HIPC(hipMemcpy(out->Out_diff_coeff, s->acc_dc, (size_t)Nt * sizeof(double), hipMemcpyDeviceToHost));
     }
     c->last_transport_s = transport_s;
     c->last_total_s = std::chrono::duration<double>(std::chrono::steady_clock::now() - tstart).count();
