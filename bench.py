@@ -130,6 +130,14 @@ def main():
     mean_s = (ms_tr / max(n_tr, 1)) * 1e-3
     achieved = per_launch / mean_s / 1e9 if mean_s > 0 else 0.0
 
+    # measured HBM bytes per launch of the transport kernel (FETCH_SIZE and
+    # WRITE_SIZE passes, scaled by a plain copy), for this workload if taken
+    traffic = None
+    tf = os.path.join(ROOT, "profiles", f"k_transport__al2o3_au11_{B}iter_traffic.txt")
+    if args.config == "config2" and os.path.exists(tf):
+        kv = dict(l.strip().split("=", 1) for l in open(tf) if "=" in l and not l.startswith("#"))
+        traffic = float(kv["traffic_bytes_per_launch"])
+
     cpu = None
     if args.cpu_iters > 0:
         sys.path.insert(0, os.path.join(ROOT, "oracle"))
@@ -162,9 +170,10 @@ def main():
                    "ion_init_kernel_ms": ms_shi, "ion_init_launches": n_shi,
                    "outside_batches_ms_per_step": sum(host_ms) / len(host_ms)},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "kernel": "k_transport", "launches": n_tr, "mean_launch_ms": ms_tr / max(n_tr, 1),
-                     "alg_bytes_per_launch": per_launch},
+                     "alg_bytes_per_launch": per_launch,
+                     "traffic_GBs": (traffic / mean_s / 1e9) if (traffic and mean_s > 0) else None},
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)
