@@ -131,6 +131,19 @@ int trk_ctx_create(const trk_material *mat, const trk_ion *ion, const trk_option
     rc |= upload(c, &b.Radiat, mat->Radiat, (size_t)s);
     rc |= upload(c, &b.Mass, mat->Mass, (size_t)mat->n_atoms);
     rc |= upload(c, &b.Pers, mat->Pers, (size_t)mat->n_atoms);
+    {
+        const int na = mat->n_atoms;
+        std::vector<double> mott((size_t)4 * na);
+        for (int a = 0; a < na; ++a) {
+            const double Z = (double)mat->Zat[a];
+            const double Mat = mat->Mass[a] * kMp;
+            mott[a] = Z;
+            mott[na + a] = trk_pow(Z, 2.0 / 3.0);
+            mott[2 * na + a] = ((Mat * kC) * kC) / kE;
+            mott[3 * na + a] = mat->Pers[a];
+        }
+        rc |= upload(c, &b.mott, mott.data(), mott.size());
+    }
     rc |= upload(c, &b.dosE, h.dosE.data(), h.dosE.size());
     rc |= upload(c, &b.dos, h.dos.data(), h.dos.size());
     rc |= upload(c, &b.dosI, h.dosI.data(), h.dosI.size());
@@ -155,6 +168,10 @@ int trk_ctx_create(const trk_material *mat, const trk_ion *ion, const trk_option
     md.dosE = b.dosE; md.dos = b.dos; md.dosI = b.dosI; md.dosM = b.dosM;
     md.Ip = b.Ip; md.Nel = b.Nel; md.Auger = b.Auger; md.Radiat = b.Radiat;
     md.atom_of = b.atom_of; md.shl_off = b.shl_off; md.Zat = b.Zat; md.Mass = b.Mass; md.Pers = b.Pers;
+    md.mott_Z = b.mott;
+    md.mott_Z23 = b.mott + mat->n_atoms;
+    md.mott_Mct2 = b.mott + 2 * mat->n_atoms;
+    md.mott_Pers = b.mott + 3 * mat->n_atoms;
     md.ion_mass = ion->Mass; md.ion_E = ion->E; md.ion_Zeff0 = h.shi_Zeff0;
     md.ion_Z = ion->Zat; md.ion_kind_zeff = ion->Kind_Zeff; md.ion_fixed_zeff = ion->fixed_Zeff;
     return 0;
@@ -167,7 +184,7 @@ void trk_ctx_destroy(trk_ctx *c)
     mc_free(c->mc);
     MatBuffers &b = c->mb;
     void *ps[] = {b.osc_off, b.atom_of, b.shl_off, b.Zat, b.oA, b.oE0, b.oG, b.phA, b.phE0, b.phG, b.k, b.Effm,
-                  b.dosE, b.dos, b.dosI, b.dosM, b.Ip, b.Nel, b.Auger, b.Radiat, b.Mass, b.Pers};
+                  b.dosE, b.dos, b.dosI, b.dosM, b.Ip, b.Nel, b.Auger, b.Radiat, b.Mass, b.Pers, b.mott};
     for (void *p : ps) if (p) (void)hipFree(p);
     for (hipEvent_t e : c->pe) if (e) (void)hipEventDestroy(e);
     if (c->stream) (void)hipStreamDestroy(c->stream);

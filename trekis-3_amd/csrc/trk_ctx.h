@@ -34,6 +34,7 @@ struct MatBuffers {
     double *dosE = nullptr, *dos = nullptr, *dosI = nullptr, *dosM = nullptr;
     double *Ip = nullptr, *Nel = nullptr, *Auger = nullptr, *Radiat = nullptr;
     double *Mass = nullptr, *Pers = nullptr;
+    double *mott = nullptr;     // [4][n_atoms]: Z, Z^(2/3), rest energy, Pers
 };
 
 struct McState;  // transport buffers, trk_mc.hip

@@ -113,9 +113,38 @@ struct PState {
     // draw (the next event looks them up again at the same energy)
     double cE, cIM, cEM;
     int cN;
+    // the particle's last Philox block (index + 1, 0 = none): draws 2k and
+    // 2k+1 come from the same block, so every other draw is a lookup
+    uint32_t cb, cw0, cw1, cw2, cw3;
 };
 
-TRK_DEV double rn(const McDev &d, PState &p) { return trk_uniform(d.seed, p.id, p.iter, p.nd++); }
+// One Philox block of a particle's stream.
+__device__ __forceinline__ void philox_block(uint32_t blk, uint64_t id, uint32_t iter, uint64_t seed, uint32_t *w)
+{
+    trk_u32x4 c;
+    c.v[0] = blk; c.v[1] = (uint32_t)id; c.v[2] = (uint32_t)(id >> 32); c.v[3] = iter;
+    trk_u32x4 r = trk_philox4x32_10(c, (uint32_t)seed, (uint32_t)(seed >> 32));
+    w[0] = r.v[0]; w[1] = r.v[1]; w[2] = r.v[2]; w[3] = r.v[3];
+}
+
+// Draw n of the particle's stream without consuming it; equals
+// trk_uniform(seed, id, iter, n) bit for bit.
+TRK_DEV double rn_at(const McDev &d, PState &p, uint32_t n)
+{
+    const uint32_t blk = n >> 1;
+    if (p.cb != blk + 1u) {
+        uint32_t w[4];
+        philox_block(blk, p.id, p.iter, d.seed, w);
+        p.cw0 = w[0]; p.cw1 = w[1]; p.cw2 = w[2]; p.cw3 = w[3];
+        p.cb = blk + 1u;
+    }
+    const uint32_t a = (n & 1u) ? p.cw2 : p.cw0;
+    const uint32_t b = (n & 1u) ? p.cw3 : p.cw1;
+    const uint64_t u = ((((uint64_t)a) << 32) | (uint64_t)b) >> 11;
+    return ((double)u + 0.5) * 0x1.0p-53;
+}
+
+TRK_DEV double rn(const McDev &d, PState &p) { return rn_at(d, p, p.nd++); }
 
 TRK_DEV void load(const PList &l, int i, PState &p, int hole)
 {
@@ -125,6 +154,7 @@ TRK_DEV void load(const PList &l, int i, PState &p, int hole)
     else { p.Ehkin = 0.0; p.Mass = 1.0; p.sh = -1; }
     p.id = l.id[i]; p.nd = l.nd[i]; p.ne = l.ne[i]; p.iter = l.iter[i];
     p.cE = TRK_NAN; p.cIM = 0.0; p.cEM = 0.0; p.cN = 0;
+    p.cb = 0u;
 }
 
 TRK_DEV void store(const PList &l, int i, const PState &p, int hole)
@@ -143,6 +173,7 @@ TRK_DEV void init_child(PState &c, uint64_t parent, uint32_t ev, uint32_t slot, 
     else { c.tn = 1.0e20; c.Mass = 1.0; c.Ehkin = 0.0; c.sh = -1; }
     c.id = trk_child_id(parent, ev, slot);
     c.nd = 0; c.ne = 0; c.iter = iter;
+    c.cb = 0u;
 }
 
 // ---------------------------------------------------------------- tallies
@@ -553,41 +584,49 @@ TRK_DEV double transfer_el(const McDev &d, double Ele, double L_tot, int hole, P
 TRK_DEV double rest_energy(double M0) { return ((M0 * kC) * kC) / kE; }
 
 // NRG_transfer_elastic_atomic with Mott_sample_mu, screening_parameter and
-// transfered_E_from_theta, Cross_sections.f90:3517-3613
-TRK_DEV double mott_transfer(const McDev &d, int atom, double Ee, double M_eff, PState &p)
+// transfered_E_from_theta, Cross_sections.f90:3517-3613, for every atom kind
+// of the target weighted by its share (MC:2395-2406).  The terms that depend
+// only on the electron energy are formed once, the per-atom constants Z^(2/3)
+// and the atom's rest energy come precomputed; the arithmetic of each term is
+// unchanged.
+TRK_DEV double mott_transfer_all(const McDev &d, double Ee, double M_eff, PState &p)
 {
-    double Z = (double)d.m.Zat[atom];
-    double Mat = d.m.Mass[atom] * kMp;
-    double RN = rn(d, p);
-    double Erest = rest_energy(kMe);
-    double fact = Ee / Erest + 1.0;
-    double v = kC * trk_sqrt(1.0 - 1.0 / (fact * fact));
-    double theta;
-    if (v < 1.0e-6) {
-        theta = 0.0;
-    } else {
-        double beta = v / kC;
-        double beta2 = beta * beta;
-        double tau = Ee / Erest;
-        double nu = (((1.7e-5 * trk_pow(Z, 2.0 / 3.0)) * (1.0 - beta2)) / beta2) *
-                    (1.13 + (((((3.76 * kAlpha) * kAlpha) / beta2) * Z) * Z) * trk_sqrt(tau / (1.0 + tau)));
-        double mu = (RN * ((2.0 * nu) + 1.0) - nu) / (RN + nu);
-        theta = trk_acos(mu);
+    const double Erest = rest_energy(kMe);
+    const double fact = Ee / Erest + 1.0;
+    const double v = kC * trk_sqrt(1.0 - 1.0 / (fact * fact));
+    const double beta = v / kC;
+    const double beta2 = beta * beta;
+    const double tau = Ee / Erest;
+    const double A1 = 1.0 - beta2;
+    const double A2 = ((3.76 * kAlpha) * kAlpha) / beta2;
+    const double S = trk_sqrt(tau / (1.0 + tau));
+    const double mc2 = rest_energy(kMe * M_eff);
+    const double Emc = Ee + mc2;
+    const double E2mc = Ee + 2.0 * mc2;
+    double dE = 0.0;
+    for (int a = 0; a < d.m.n_atoms; ++a) {
+        const double Z = d.m.mott_Z[a];
+        const double Mct2 = d.m.mott_Mct2[a];
+        double RN = rn(d, p);
+        double theta;
+        if (v < 1.0e-6) {
+            theta = 0.0;
+        } else {
+            double nu = (((1.7e-5 * d.m.mott_Z23[a]) * A1) / beta2) * (1.13 + ((A2 * Z) * Z) * S);
+            double mu = (RN * ((2.0 * nu) + 1.0) - nu) / (RN + nu);
+            theta = trk_acos(mu);
+        }
+        double ct = trk_cos(theta);
+        double ct2 = ct * ct;
+        double st2 = 1.0 - ct2;
+        double EmcMc = Emc + Mct2;
+        double W1 = ((Emc * st2) + Mct2) - (ct * trk_sqrt((Mct2 * Mct2) - ((mc2 * mc2) * st2)));
+        double W2 = (Ee * E2mc) / ((EmcMc * EmcMc) - ((Ee * E2mc) * ct2));
+        dE = dE + (W1 * W2) * d.m.mott_Pers[a];
     }
-    double mc2 = rest_energy(kMe * M_eff);
-    double Mct2 = rest_energy(Mat);
-    double ct = trk_cos(theta);
-    double ct2 = ct * ct;
-    double st2 = 1.0 - ct2;
-    double Emc = Ee + mc2;
-    double E2mc = Ee + 2.0 * mc2;
-    double EmcMc = Emc + Mct2;
-    double W1 = ((Emc * st2) + Mct2) - (ct * trk_sqrt((Mct2 * Mct2) - ((mc2 * mc2) * st2)));
-    double W2 = (Ee * E2mc) / ((EmcMc * EmcMc) - ((Ee * E2mc) * ct2));
-    return W1 * W2;
+    return dE / d.m.sumPers;
 }
 
-// cos_theta_from_W, MC:1275-1299
 TRK_DEV double cos_theta_W(const McDev &d, double E, double W, double M_in, PState &p)
 {
     double Ein = rest_energy(M_in);
@@ -846,9 +885,7 @@ TRK_DEV int electron_event(const McDev &d, PState &e, int rep)
         if (d.m.kind_of_EMFP == 1) {
             dE = transfer_el(d, Eel, EMt, 0, e);
         } else {
-            dE = 0.0;
-            for (int a = 0; a < d.m.n_atoms; ++a) dE = dE + mott_transfer(d, a, Eel, 1.0, e) * d.m.Pers[a];
-            dE = dE / d.m.sumPers;
+            dE = mott_transfer_all(d, Eel, 1.0, e);
         }
         double mu = cos_theta_W(d, Eel, dE, 1.0 * kMe, e);
         theta = trk_acos(mu);
@@ -950,9 +987,7 @@ TRK_DEV int hole_event(const McDev &d, PState &h, int rep)
             if (d.m.kind_of_EMFP == 1) {
                 dE = transfer_el(d, Eel, HEt, 1, h);
             } else {
-                dE = 0.0;
-                for (int a = 0; a < d.m.n_atoms; ++a) dE = dE + mott_transfer(d, a, Eel, h.Mass, h) * d.m.Pers[a];
-                dE = dE / d.m.sumPers;
+                dE = mott_transfer_all(d, Eel, h.Mass, h);
             }
             double mu = cos_theta_W(d, Eel, dE, h.Mass * kMe, h);
             hth1 = trk_acos(mu);
@@ -1158,6 +1193,7 @@ __global__ void __launch_bounds__(64) k_ion_init(McDev d, int nb, uint32_t first
     I.E = d.m.ion_E; I.t0 = 0.0; I.X = 0.0; I.Y = 0.0; I.Z = 0.0; I.th = 0.0; I.ph = 0.0;
     I.Ehkin = 0.0; I.Mass = d.m.ion_Zeff0; I.sh = -1;   // Mass holds Zeff for the ion
     I.id = 0; I.nd = 0; I.ne = 0; I.iter = first_iter + (uint32_t)b;
+    I.cb = 0u;
     const double MSHI = d.m.ion_mass * kMp;
     double SL = nfp_2d(d.ES, d.shi_path, d.nS, I.E);
     SL = -SL * trk_log(rn(d, I));
@@ -1225,7 +1261,7 @@ TRK_DEV int peek_category(const McDev &d, PState &p, int kind)
     if (kind == K_PH) return C_PHOT;
     if (kind == K_ION) return C_ION;
     if (kind == K_HO && p.sh != d.m.vb) return C_DEEP;
-    const double RN = trk_uniform(d.seed, p.id, p.iter, p.nd);
+    const double RN = rn_at(d, p, p.nd);
     if (kind == K_EL) {
         if (!(p.E == p.cE)) {
             double lE = trk_log(p.E);

@@ -26,6 +26,9 @@ struct MatDev {
     const int *atom_of, *shl_off;
     const int *Zat;
     const double *Mass, *Pers;
+    // per-atom constants of the screened-Rutherford (Mott) sampling, computed
+    // once on the host: Z, Z^(2/3), the atom's rest energy [eV], Pers
+    const double *mott_Z, *mott_Z23, *mott_Mct2, *mott_Pers;
     double ion_mass, ion_E, ion_Zeff0;
     int ion_Z, ion_kind_zeff;
     double ion_fixed_zeff;
