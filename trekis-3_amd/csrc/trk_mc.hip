@@ -1294,7 +1294,7 @@ constexpr int kMaxWait = 8;
 // Events one particle may take in one round.  A particle with more left is
 // written to the next round's list; this bounds the tail of every launch by
 // the cost of kRoundEvents events instead of the longest history.
-constexpr uint32_t kRoundEvents = 256;
+constexpr uint32_t kRoundEvents = 512;
 
 // One round of transport.  Items [0, n_e) are electrons, then holes, photons
 // and ions.  A lane takes the next item from a shared cursor whenever its
@@ -1493,6 +1493,79 @@ __global__ void k_diff_coeff(double *D, const unsigned long long *dcs, const int
         }
     }
     D[i] = v;
+}
+
+// ---------------------------------------------------------------- tally reduction
+// Sums the R replicas of every tally cell and takes the prefix over time of
+// the difference arrays, in exact integer arithmetic, leaving the result in
+// replica 0: counts as int64, energies as int128 (low word, high word) in
+// units of 2^-40 eV.  One thread per column: electrons and photons by radial
+// bin, the lattice energy by radial bin (per node, no prefix), holes by
+// (radial bin, shell).
+TRK_DEV __int128 fx_read(const unsigned long long *p)
+{
+    return (__int128)(long long)p[1] * ((__int128)1 << 32) + (__int128)p[0];
+}
+
+TRK_DEV void i128_store(unsigned long long *p, __int128 v)
+{
+    p[0] = (unsigned long long)v;
+    p[1] = (unsigned long long)(v >> 64);
+}
+
+__global__ void k_tally_reduce(Tally T)
+{
+    const int NR = T.NR, Nt = T.Nt, nsh = T.nsh, R = T.R;
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const size_t plane = (size_t)Nt * NR;
+    if (t < 2 * NR) {                       // electrons, photons
+        const int j = (int)(t % NR);
+        unsigned long long *cnt = (t < NR) ? T.cnt_e : T.cnt_p;
+        unsigned long long *en = (t < NR) ? T.E_e : T.E_p;
+        long long rc = 0;
+        __int128 re = 0;
+        for (int i = 0; i < Nt; ++i) {
+            const size_t a = (size_t)i * NR + j;
+            long long sc = 0;
+            __int128 se = 0;
+            for (int r = 0; r < R; ++r) {
+                sc += (long long)cnt[r * plane + a];
+                se += fx_read(en + 2 * (r * plane + a));
+            }
+            rc += sc;
+            re += se;
+            cnt[a] = (unsigned long long)rc;
+            i128_store(en + 2 * a, re);
+        }
+    } else if (t < 3 * NR) {                // lattice energy per node
+        const int j = (int)(t - 2 * NR);
+        for (int i = 0; i < Nt; ++i) {
+            const size_t a = (size_t)i * NR + j;
+            __int128 se = 0;
+            for (int r = 0; r < R; ++r) se += fx_read(T.Elat + 2 * (r * plane + a));
+            i128_store(T.Elat + 2 * a, se);
+        }
+    } else if (t < 3 * NR + (int64_t)NR * nsh) {   // holes
+        const int64_t q = t - 3 * NR;
+        const int j = (int)(q / nsh), sh = (int)(q % nsh);
+        const size_t hplane = plane * nsh;
+        long long rc = 0;
+        __int128 rE = 0, rK = 0;
+        for (int i = 0; i < Nt; ++i) {
+            const size_t a = ((size_t)i * NR + j) * nsh + sh;
+            long long sc = 0;
+            __int128 sE = 0, sK = 0;
+            for (int r = 0; r < R; ++r) {
+                sc += (long long)T.cnt_h[r * hplane + a];
+                sE += fx_read(T.E_h + 2 * (r * hplane + a));
+                sK += fx_read(T.Ek_h + 2 * (r * hplane + a));
+            }
+            rc += sc; rE += sE; rK += sK;
+            T.cnt_h[a] = (unsigned long long)rc;
+            i128_store(T.E_h + 2 * a, rE);
+            i128_store(T.Ek_h + 2 * a, rK);
+        }
+    }
 }
 
 // ---------------------------------------------------------------- host side
@@ -1929,70 +2002,62 @@ int run_mc(trk_ctx *c, int64_t nmc, uint64_t seed, int64_t first_iter, const dou
         if (c->trace_n_out) *c->trace_n_out = (int64_t)tn;
     }
 
-    // finalize: sum replicas, prefix over time, convert, scale by Out_V
-    std::vector<unsigned long long> he(cell), hEe(2 * cell), hh(cell * nsh), hEh(2 * cell * nsh), hEk(2 * cell * nsh),
-        hp(cell), hEp(2 * cell), hEl(2 * cell);
-    HIPC(hipMemcpy(he.data(), T.cnt_e, cell * 8, hipMemcpyDeviceToHost));
-    HIPC(hipMemcpy(hEe.data(), T.E_e, cell * 16, hipMemcpyDeviceToHost));
-    HIPC(hipMemcpy(hh.data(), T.cnt_h, cell * nsh * 8, hipMemcpyDeviceToHost));
-    HIPC(hipMemcpy(hEh.data(), T.E_h, cell * nsh * 16, hipMemcpyDeviceToHost));
-    HIPC(hipMemcpy(hEk.data(), T.Ek_h, cell * nsh * 16, hipMemcpyDeviceToHost));
-    HIPC(hipMemcpy(hp.data(), T.cnt_p, cell * 8, hipMemcpyDeviceToHost));
-    HIPC(hipMemcpy(hEp.data(), T.E_p, cell * 16, hipMemcpyDeviceToHost));
-    HIPC(hipMemcpy(hEl.data(), T.Elat, cell * 16, hipMemcpyDeviceToHost));
+    // finalize: replicas summed and time prefix taken on the device (exact
+    // integers); the host converts, scales by Out_V and forms the totals
+    {
+        const int64_t ncol = 3 * (int64_t)NR + (int64_t)NR * nsh;
+        k_tally_reduce<<<(unsigned)((ncol + 127) / 128), 128, 0, st>>>(T);
+        HIPC(hipGetLastError());
+    }
     const size_t plane = (size_t)Nt * NR;
-    auto cidx = [&](int r, int i, int j) { return ((size_t)r * Nt + i) * NR + j; };
+    std::vector<unsigned long long> he(plane), hEe(2 * plane), hh(plane * nsh), hEh(2 * plane * nsh),
+        hEk(2 * plane * nsh), hp(plane), hEp(2 * plane), hEl(2 * plane);
+    HIPC(hipMemcpyAsync(he.data(), T.cnt_e, plane * 8, hipMemcpyDeviceToHost, st));
+    HIPC(hipMemcpyAsync(hEe.data(), T.E_e, plane * 16, hipMemcpyDeviceToHost, st));
+    HIPC(hipMemcpyAsync(hh.data(), T.cnt_h, plane * nsh * 8, hipMemcpyDeviceToHost, st));
+    HIPC(hipMemcpyAsync(hEh.data(), T.E_h, plane * nsh * 16, hipMemcpyDeviceToHost, st));
+    HIPC(hipMemcpyAsync(hEk.data(), T.Ek_h, plane * nsh * 16, hipMemcpyDeviceToHost, st));
+    HIPC(hipMemcpyAsync(hp.data(), T.cnt_p, plane * 8, hipMemcpyDeviceToHost, st));
+    HIPC(hipMemcpyAsync(hEp.data(), T.E_p, plane * 16, hipMemcpyDeviceToHost, st));
+    HIPC(hipMemcpyAsync(hEl.data(), T.Elat, plane * 16, hipMemcpyDeviceToHost, st));
+    HIPC(hipStreamSynchronize(st));
+    auto i128 = [](const unsigned long long *p) { return (__int128)(((unsigned __int128)p[1] << 64) | p[0]); };
     const int na = h.m.n_atoms;
-    std::vector<__int128> run_e(NR, 0), run_Ee(NR, 0), run_p(NR, 0), run_Ep(NR, 0);
-    std::vector<__int128> run_h((size_t)NR * nsh, 0), run_Eh((size_t)NR * nsh, 0), run_Ek((size_t)NR * nsh, 0);
     __int128 at_cum = 0;
-    (void)plane;
+    std::vector<__int128> shE(nsh);
     for (int i = 0; i < Nt; ++i) {
         __int128 tot_e = 0, tot_Ee = 0, tot_p = 0, tot_Ep = 0, row_lat = 0;
-        std::vector<__int128> shE(nsh, 0);
+        for (int s2 = 0; s2 < nsh; ++s2) shE[s2] = 0;
         for (int j = 0; j < NR; ++j) {
-            __int128 se = 0, sEe = 0, sp = 0, sEp = 0, sl = 0;
-            for (int r = 0; r < R; ++r) {
-                size_t a = cidx(r, i, j);
-                se += (__int128)(long long)he[a];
-                sEe += fx_value(&hEe[2 * a]);
-                sp += (__int128)(long long)hp[a];
-                sEp += fx_value(&hEp[2 * a]);
-                sl += fx_value(&hEl[2 * a]);
-            }
-            run_e[j] += se; run_Ee[j] += sEe; run_p[j] += sp; run_Ep[j] += sEp;
-            tot_e += run_e[j]; tot_Ee += run_Ee[j]; tot_p += run_p[j]; tot_Ep += run_Ep[j]; row_lat += sl;
+            const size_t a = (size_t)i * NR + j;
+            const long long ne_ = (long long)he[a], np_ = (long long)hp[a];
+            const __int128 Ee = i128(&hEe[2 * a]), Ep = i128(&hEp[2 * a]), El = i128(&hEl[2 * a]);
+            tot_e += ne_; tot_Ee += Ee; tot_p += np_; tot_Ep += Ep; row_lat += El;
             const double V = Out_V[j];
             const size_t o = (size_t)i + (size_t)Nt * j;
-            if (out->Out_ne) out->Out_ne[o] += (double)(long long)run_e[j] * V;
-            if (out->Out_Ee) out->Out_Ee[o] += fx_to_double(run_Ee[j]) * V;
-            if (out->Out_nphot) out->Out_nphot[o] += (double)(long long)run_p[j] * V;
-            if (out->Out_Ephot) out->Out_Ephot[o] += fx_to_double(run_Ep[j]) * V;
-            if (out->Out_Elat) out->Out_Elat[o] += fx_to_double(sl) * V;
-            for (int s = 0; s < nsh; ++s) {
-                __int128 sh_ = 0, sEh = 0, sEk = 0;
-                for (int r = 0; r < R; ++r) {
-                    size_t a = cidx(r, i, j) * nsh + s;
-                    sh_ += (__int128)(long long)hh[a];
-                    sEh += fx_value(&hEh[2 * a]);
-                    sEk += fx_value(&hEk[2 * a]);
-                }
-                size_t q = (size_t)j * nsh + s;
-                run_h[q] += sh_; run_Eh[q] += sEh; run_Ek[q] += sEk;
-                shE[s] += run_Eh[q] + run_Ek[q];
-                int a = c->hm.atom_of[s], sl = s - c->hm.shl_off[a];
-                size_t oh = (size_t)i + (size_t)Nt * (j + (size_t)NR * (a + (size_t)na * sl));
-                if (out->Out_nh) out->Out_nh[oh] += (double)(long long)run_h[q] * V;
-                if (out->Out_Eh) out->Out_Eh[oh] += fx_to_double(run_Eh[q]) * V;
-                if (out->Out_Ehkin) out->Out_Ehkin[oh] += fx_to_double(run_Ek[q]) * V;
+            if (out->Out_ne) out->Out_ne[o] += (double)ne_ * V;
+            if (out->Out_Ee) out->Out_Ee[o] += fx_to_double(Ee) * V;
+            if (out->Out_nphot) out->Out_nphot[o] += (double)np_ * V;
+            if (out->Out_Ephot) out->Out_Ephot[o] += fx_to_double(Ep) * V;
+            if (out->Out_Elat) out->Out_Elat[o] += fx_to_double(El) * V;
+            for (int s2 = 0; s2 < nsh; ++s2) {
+                const size_t q = a * nsh + s2;
+                const long long nh_ = (long long)hh[q];
+                const __int128 Eh = i128(&hEh[2 * q]), Ek = i128(&hEk[2 * q]);
+                shE[s2] += Eh + Ek;
+                int at = c->hm.atom_of[s2], sl = s2 - c->hm.shl_off[at];
+                size_t oh = (size_t)i + (size_t)Nt * (j + (size_t)NR * (at + (size_t)na * sl));
+                if (out->Out_nh) out->Out_nh[oh] += (double)nh_ * V;
+                if (out->Out_Eh) out->Out_Eh[oh] += fx_to_double(Eh) * V;
+                if (out->Out_Ehkin) out->Out_Ehkin[oh] += fx_to_double(Ek) * V;
             }
         }
         at_cum += row_lat;
         __int128 hsum = 0;
-        for (int s = 0; s < nsh; ++s) {
-            hsum += shE[s];
-            int a = c->hm.atom_of[s], sl = s - c->hm.shl_off[a];
-            if (out->Out_E_h) out->Out_E_h[(size_t)i + (size_t)Nt * (a + (size_t)na * sl)] += fx_to_double(shE[s]);
+        for (int s2 = 0; s2 < nsh; ++s2) {
+            hsum += shE[s2];
+            int at = c->hm.atom_of[s2], sl = s2 - c->hm.shl_off[at];
+            if (out->Out_E_h) out->Out_E_h[(size_t)i + (size_t)Nt * (at + (size_t)na * sl)] += fx_to_double(shE[s2]);
         }
         if (out->Out_tot_Ne) out->Out_tot_Ne[i] += (double)(long long)tot_e;
         if (out->Out_E_e) out->Out_E_e[i] += fx_to_double(tot_Ee);
