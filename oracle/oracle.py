@@ -60,6 +60,11 @@ def load():
         "or_prepare_dos": (c_int, [DP, DP, c_int32, c_double, c_double, c_double, DP, DP, DP, DP, DP, DP, DP, c_int32]),
         "or_time_nodes": (c_int, [c_void_p, P(c_int32), DP, c_int32]),
         "or_constant": (c_double, [c_int]),
+        "or_sample_transfer": (c_double, [c_void_p, c_int, c_int, c_double, c_double]),
+        "or_mott_theta": (c_double, [c_void_p, c_int, c_double, c_double, DP]),
+        "or_mott_W": (c_double, [c_void_p, c_int, c_double, c_double, c_double]),
+        "or_cos_theta_W": (c_double, [c_void_p, c_double, c_double, c_double]),
+        "or_auger_pick": (c_int, [c_void_p, c_double, c_double, c_int]),
         "or_run_mc": (c_int, [c_void_p, c_int64, c_uint64, c_int64, DP, DP, c_int32, P(abi.Tallies), DP, c_int64,
                               P(c_int64)]),
     }

@@ -69,6 +69,13 @@ int    or_prepare_dos(const double *raw_E, const double *raw_DOS, int32_t n_raw,
                       double *E, double *DOS, double *int_DOS, double *k, double *Eff_m,
                       double *k_inv, double *Eff_m_inv, int32_t M);
 int    or_time_nodes(const or_ctx *c, int32_t *Nt, double *tim_glob, int32_t cap);
+/* inverse-CDF energy-transfer samplers for a given random number (tests) */
+double or_sample_transfer(or_ctx *c, int kind, int shell, double Ele, double RN);
+/* elastic kinematics and decay-shell choice (tests) */
+double or_mott_theta(const or_ctx *c, int atom, double Ee, double RN, double *nu_out);
+double or_mott_W(const or_ctx *c, int atom, double Ee, double theta, double M_eff);
+double or_cos_theta_W(const or_ctx *c, double E, double W, double M_in);
+int    or_auger_pick(const or_ctx *c, double NRG, double frac, int second);
 double or_constant(int which);
 
 /* Monte Carlo (do_Monte_Carlo in the reference's global time order).  If

@@ -235,11 +235,10 @@ static double hole_mass_tab(const or_ctx *c, double Ele)
     return c->dosM[j - 1];
 }
 
-/* Electron_energy_transfer_inelastic, CS:1793-1871 (tabulated branch) */
-static double e_transfer_inel(or_mc *S, double Ele, int shell, double L_tot, int hole, uint64_t id, uint32_t *nd)
+/* Electron_energy_transfer_inelastic, CS:1793-1871 (tabulated branch), for a
+ * given random number */
+static double transfer_inel_rn(or_ctx *c, double Ele, int shell, double L_tot, int hole, double RN)
 {
-    or_ctx *c = S->c;
-    double RN = rn(S, id, nd);
     double L_need = L_tot / RN;
     double Emin = c->m.Ip[shell];
     double Egap = c->Egap;
@@ -266,6 +265,12 @@ static double e_transfer_inel(or_mc *S, double Ele, int shell, double L_tot, int
     if (E > Emax) E = Emax;
     if (E != E) E = Emin;
     return E;
+}
+
+static double e_transfer_inel(or_mc *S, double Ele, int shell, double L_tot, int hole, uint64_t id, uint32_t *nd)
+{
+    double RN = rn(S, id, nd);
+    return transfer_inel_rn(S->c, Ele, shell, L_tot, hole, RN);
 }
 
 /* Electron_energy_transfer_elastic, CS:2285-2428 (tabulated branch) */
@@ -337,6 +342,40 @@ static double cos_theta_W(or_mc *S, double E, double W, double M_in, double mt, 
         mu = COS(th);
     }
     return mu;
+}
+
+/* Test hooks for the elastic kinematics and the decay-shell choice. */
+double or_mott_theta(const or_ctx *c, int atom, double Ee, double RN, double *nu_out)
+{
+    double Z = (double)c->m.Zat[atom];
+    double Erest = or_rest_energy(OR_ME);
+    double fact = Ee / Erest + 1.0;
+    double v = OR_CVEL * SQRT(1.0 - 1.0 / (fact * fact));
+    double beta = v / OR_CVEL, beta2 = beta * beta, tau = Ee / Erest;
+    double nu = (((1.7e-5 * trk_pow(Z, 2.0 / 3.0)) * (1.0 - beta2)) / beta2) *
+                (1.13 + (((((3.76 * OR_ALPHA) * OR_ALPHA) / beta2) * Z) * Z) * SQRT(tau / (1.0 + tau)));
+    if (nu_out) *nu_out = nu;
+    return ACOS((RN * ((2.0 * nu) + 1.0) - nu) / (RN + nu));
+}
+
+double or_mott_W(const or_ctx *c, int atom, double Ee, double theta, double M_eff)
+{
+    double Mct2 = or_rest_energy(c->m.Mass[atom] * OR_MP);
+    double mc2 = or_rest_energy(OR_ME * M_eff);
+    double ct = COS(theta), ct2 = ct * ct, st2 = 1.0 - ct2;
+    double Emc = Ee + mc2, E2mc = Ee + 2.0 * mc2, EmcMc = Emc + Mct2;
+    double W1 = ((Emc * st2) + Mct2) - (ct * SQRT((Mct2 * Mct2) - ((mc2 * mc2) * st2)));
+    double W2 = (Ee * E2mc) / ((EmcMc * EmcMc) - ((Ee * E2mc) * ct2));
+    return W1 * W2;
+}
+
+double or_cos_theta_W(const or_ctx *c, double E, double W, double M_in)
+{
+    double Ein = or_rest_energy(M_in), Et = or_rest_energy(c->Mtarget);
+    double E2mc = E + 2.0 * Ein, EmW = E - W;
+    double W1 = (E * E2mc) - (W * ((E + Ein) + Et));
+    double W2 = ((E * E2mc) * EmW) * (E2mc - W);
+    return (W2 > 0.0) ? W1 / SQRT(W2) : 0.0;
 }
 
 /* New_Angles_both, MC:1328-1360 */
@@ -439,6 +478,15 @@ static int auger_choose(const or_ctx *c, double NRG, double Shel, int second, in
     }
     return found;
 }
+
+/* Test hook: the shell the cumulative choice picks for a random fraction. */
+int or_auger_pick(const or_ctx *c, double NRG, double frac, int second)
+{
+    double coun = auger_count(c, NRG, second);
+    if (!(coun > 0.0)) return -1;
+    return auger_choose(c, NRG, frac * coun, second, -1);
+}
+
 
 /* ---------------------------------------------------------------- kinematics */
 static double vel_el(double E) { return SQRT(((2.0 * E) * OR_E) / OR_ME); }
@@ -564,13 +612,11 @@ static void tally_elat(or_mc *S, double X, double Y, double dE)
 
 /* ---------------------------------------------------------------- SHI */
 /* SHI_energy_transfer, MC:1719-1780 (CDF branch) */
-static double shi_transfer(or_mc *S, int shell, uint64_t id, uint32_t *nd)
+static double shi_transfer_rn(const or_ctx *c, int shell, double RN)
 {
-    or_ctx *c = S->c;
     const or_table *t = &c->tab[TRK_TAB_SHI_DIFF];
     int N = t->dn[shell];
     const double *E = t->dE[shell], *L = t->dL[shell];
-    double RN = rn(S, id, nd);
     double Ecur = c->m.Ip[shell];
     int M = or_find_mono_1d(E, N, c->m.Ip[shell]);
     double dL;
@@ -595,6 +641,24 @@ static double shi_transfer(or_mc *S, int shell, uint64_t id, uint32_t *nd)
     if (Nt > M)
         return or_interpolate(5, 1.0 / L[Nt - 2], 1.0 / L[Nt - 1], E[Nt - 2], E[Nt - 1], Tot);
     return c->m.Ip[shell];
+}
+
+static double shi_transfer(or_mc *S, int shell, uint64_t id, uint32_t *nd)
+{
+    double RN = rn(S, id, nd);
+    return shi_transfer_rn(S->c, shell, RN);
+}
+
+/* Test hooks: the inverse-CDF samplers for a given random number.  kind 0
+ * electron inelastic (shell's own mean free path as the reference passes it,
+ * MC:2329-2331), 1 VB-hole inelastic, 2 ion (SHI_energy_transfer). */
+double or_sample_transfer(or_ctx *c, int kind, int shell, double Ele, double RN)
+{
+    if (kind == 2) return shi_transfer_rn(c, shell, RN);
+    const or_table *t = &c->tab[kind == 0 ? TRK_TAB_E_INEL : TRK_TAB_H_INEL];
+    int s = (kind == 0) ? shell : c->vb;
+    double L = or_nfp_1d(c, Ele, t->E, t->L[s], t->n);
+    return transfer_inel_rn(c, Ele, s, L, kind == 1, RN);
 }
 
 /* SHI_Monte_Carlo, MC:2153-2249 */
