@@ -67,6 +67,7 @@ def load():
         "or_auger_pick": (c_int, [c_void_p, c_double, c_double, c_int]),
         "or_run_mc": (c_int, [c_void_p, c_int64, c_uint64, c_int64, DP, DP, c_int32, P(abi.Tallies), DP, c_int64,
                               P(c_int64)]),
+        "or_run_mc_threads": (c_int, [c_void_p, c_int64, c_uint64, c_int64, DP, DP, c_int32, P(abi.Tallies), c_int]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)
@@ -153,6 +154,13 @@ class Oracle:
                         break
                     self.lib.or_row_set(self.h, kind, s, i, dp(hw), dp(lam), len(hw))
         self.lib.or_finish_tables(self.h)
+
+    def run_mc_threads(self, nmc, seed, Out_R, Out_V, tallies, threads, first_iter=0):
+        rc = self.lib.or_run_mc_threads(self.h, nmc, seed, first_iter, dp(Out_R), dp(Out_V), len(Out_R),
+                                        ctypes.byref(tallies.s), threads)
+        if rc:
+            raise RuntimeError(self.lib.or_error(self.h).decode())
+        return tallies
 
     def run_mc(self, nmc, seed, Out_R, Out_V, tallies, first_iter=0, trace_cap=0):
         trace = np.zeros((max(trace_cap, 1), 12))

@@ -86,6 +86,11 @@ int or_run_mc(or_ctx *c, int64_t nmc, uint64_t seed, int64_t first_iter,
               const double *Out_R, const double *Out_V, int32_t NR, trk_tallies *out,
               double *trace, int64_t trace_cap, int64_t *trace_n);
 
+/* The same on OpenMP threads over iterations (timing of the CPU path on all
+ * cores; sums not in iteration order, Out_diff_coeff not produced). */
+int or_run_mc_threads(or_ctx *c, int64_t nmc, uint64_t seed, int64_t first_iter,
+                      const double *Out_R, const double *Out_V, int32_t NR, trk_tallies *out, int threads);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1293,3 +1293,56 @@ int or_run_mc(or_ctx *c, int64_t nmc, uint64_t seed, int64_t first_iter,
     if (c->err_numeric) { snprintf(c->err, sizeof c->err, "NaN reached a table search"); return TRK_ERR_NUMERIC; }
     return 0;
 }
+
+/* The same iterations spread over OpenMP threads, each with private tally
+ * copies summed at the end (the reference's OpenMP reduction over
+ * iterations, MC:395-415).  Sums are not in iteration order, so results are
+ * not bitwise those of or_run_mc; Out_diff_coeff (an order-dependent
+ * recurrence) is not produced.  Used to time the CPU path on all cores. */
+int or_run_mc_threads(or_ctx *c, int64_t nmc, uint64_t seed, int64_t first_iter,
+                      const double *Out_R, const double *Out_V, int32_t NR, trk_tallies *out, int threads)
+{
+    if (!c->tables_ready) { snprintf(c->err, sizeof c->err, "tables not built"); return TRK_ERR_STATE; }
+    if (out->Nt != c->Nt) { snprintf(c->err, sizeof c->err, "Nt mismatch %d vs %d", out->Nt, c->Nt); return TRK_ERR_ARG; }
+    const int Nt = out->Nt, na = c->m.n_atoms, ms = out->max_shells, nd = out->n_dos;
+    const size_t n2 = (size_t)Nt * NR, nh = n2 * na * ms, n1 = (size_t)Nt, ns = (size_t)Nt * na * ms;
+    const size_t nth = (size_t)(Nt + 1) * 180, ndos = (size_t)Nt * (nd > 0 ? nd : 1);
+    int64_t ev[8] = {0};
+#pragma omp parallel num_threads(threads > 0 ? threads : 1)
+    {
+        trk_tallies L = *out;
+        double **dst[] = {&L.Out_ne, &L.Out_Ee, &L.Out_nphot, &L.Out_Ephot, &L.Out_Ee_vs_E, &L.Out_Elat,
+                          &L.Out_nh, &L.Out_Eh, &L.Out_Ehkin, &L.Out_tot_Ne, &L.Out_tot_Nphot, &L.Out_tot_E,
+                          &L.Out_E_e, &L.Out_E_phot, &L.Out_E_at, &L.Out_E_h, &L.Out_theta, &L.Out_theta_h,
+                          &L.Out_Eh_vs_E};
+        double *const src[] = {out->Out_ne, out->Out_Ee, out->Out_nphot, out->Out_Ephot, out->Out_Ee_vs_E,
+                               out->Out_Elat, out->Out_nh, out->Out_Eh, out->Out_Ehkin, out->Out_tot_Ne,
+                               out->Out_tot_Nphot, out->Out_tot_E, out->Out_E_e, out->Out_E_phot, out->Out_E_at,
+                               out->Out_E_h, out->Out_theta, out->Out_theta_h, out->Out_Eh_vs_E};
+        const size_t len[] = {n2, n2, n2, n2, n2, n2, nh, nh, nh, n1, n1, n1, n1, n1, n1, ns, nth, nth, ndos};
+        const int na_ = (int)(sizeof len / sizeof len[0]);
+        for (int k = 0; k < na_; ++k) *dst[k] = src[k] ? (double *)calloc(len[k], sizeof(double)) : NULL;
+        L.Out_diff_coeff = NULL;
+        for (int k = 0; k < 8; ++k) L.n_events[k] = 0;
+        or_mc S;
+        memset(&S, 0, sizeof S);
+        S.c = c; S.seed = seed; S.R = Out_R; S.V = Out_V; S.NR = NR; S.out = &L;
+#pragma omp for schedule(dynamic, 1)
+        for (int64_t it = 0; it < nmc; ++it) {
+            S.iter = (uint32_t)(first_iter + it);
+            one_iteration(&S);
+        }
+#pragma omp critical
+        {
+            for (int k = 0; k < na_; ++k)
+                if (src[k])
+                    for (size_t i = 0; i < len[k]; ++i) src[k][i] += (*dst[k])[i];
+            for (int k = 0; k < 8; ++k) ev[k] += S.n_events[k];
+        }
+        for (int k = 0; k < na_; ++k) free(*dst[k]);
+        free(S.el); free(S.ho); free(S.ph); free(S.heap);
+    }
+    for (int k = 0; k < 8; ++k) out->n_events[k] += ev[k];
+    if (c->err_numeric) { snprintf(c->err, sizeof c->err, "NaN reached a table search"); return TRK_ERR_NUMERIC; }
+    return 0;
+}

@@ -109,3 +109,14 @@ def test_iterations_are_independent(runs):
     for k in ("Out_ne", "Out_Ee", "Out_tot_E", "Out_Elat", "Out_Eh"):
         np.testing.assert_allclose(p0[k] + p1[k], whole[k], rtol=1e-11, atol=1e-9)
     assert ((p0.n_events + p1.n_events) == whole.n_events).all()
+
+
+def test_threaded_run_matches_sequential(runs):
+    """The all-core CPU timing path (OpenMP over iterations, private tallies)
+    gives the sequential run's events and tallies up to summation order."""
+    o, run, R, V, mat = runs["o"], runs["run"], runs["R"], runs["V"], runs["mat"]
+    a = o.run_mc(4, run["seed"], R, V, tg.TallySet(o.Nt, len(R), mat))
+    b = o.run_mc_threads(4, run["seed"], R, V, tg.TallySet(o.Nt, len(R), mat), threads=4)
+    assert (a.n_events == b.n_events).all()
+    for k in ("Out_ne", "Out_Ee", "Out_nh", "Out_Elat", "Out_tot_E", "Out_theta"):
+        np.testing.assert_allclose(b[k], a[k], rtol=1e-11, atol=1e-11 * abs(a[k]).max())
