@@ -1,14 +1,20 @@
 """Benchmark of the particle-history path on one MI355X.
 
 Workload: the Al2O3-like target (Egap 8.8 eV, 4 VB + Al L + O K oscillator
-sets, 3.95 g/cm^3, 100 A layer) hit by Au (Z 79) at 11 MeV/u, FP64, tallies on
-1000 time nodes (0-1000 fs, dt 1 fs) x 100 log radial bins (1-1000 A).  Tables
-are built once before timing (resident in HBM).  One step = trk_run_mc over
+sets, 3.95 g/cm^3, 100 A layer) hit by Au (Z 79) at 11 MeV/u, tallies on 1000
+time nodes (0-1000 fs, dt 1 fs) x 100 log radial bins (1-1000 A).  Tables are
+built once before timing (resident in HBM).  One step = trk_run_mc over
 --iters-per-step Monte Carlo iterations (whole ion histories with all their
 cascades), with distinct iteration numbers so every step draws fresh random
 streams.  The metric is collision events per second (every SHI ionisation,
 electron/hole elastic or inelastic collision, Auger or radiative decay and
 photoabsorption counts once).
+
+The value is the mixed-precision path (precision 1: FP32 sampling, FP64
+state, ledger and tallies).  The same iterations, with the same random
+streams, are then run on the FP64 path (precision 0, which replays the oracle
+bit for bit): its events/s, the ratio and the error of every radial tally of
+the mixed path against it are reported beside the value.
 
 Also reported:
   roofline      the transport kernel's algorithmic bytes (per-event state
@@ -52,12 +58,97 @@ def transport_bytes(ev, nsh, snaps_e, snaps_h, snaps_p, phonon_rows):
     return b
 
 
+RADIAL = ["Out_ne", "Out_Ee", "Out_nh", "Out_Eh", "Out_Ehkin", "Out_Elat", "Out_nphot", "Out_Ephot"]
+PHOTON = ("Out_nphot", "Out_Ephot")
+MIN_PHOTON_COUNTS = 1.0e4
+
+
+def tally_errors(ref, mix, min_frac=1.0e-2):
+    """Error of each radial tally (time x radius; hole tallies summed over
+    shells) of `mix` against `ref`: the largest relative difference over the
+    bins holding at least min_frac of the tally's peak, and the relative L1
+    difference over all bins.  Photon tallies enter the overall maximum only
+    when the reference holds MIN_PHOTON_COUNTS photon counts or more (fewer
+    leave single bins dominated by a handful of photons)."""
+    out, worst = {}, 0.0
+    photons = float(ref["Out_nphot"].sum())
+    for k in RADIAL:
+        a, b = ref[k], mix[k]
+        if a.ndim == 4:
+            a, b = a.sum(axis=(2, 3)), b.sum(axis=(2, 3))
+        peak = float(np.abs(a).max())
+        if peak == 0.0:
+            out[k] = {"bins": 0}
+            continue
+        sel = np.abs(a) >= min_frac * peak
+        rel = np.abs(b - a)[sel] / np.abs(a)[sel]
+        e = {"max_rel": float(rel.max()), "l1_rel": float(np.abs(b - a).sum() / np.abs(a).sum()),
+             "bins": int(sel.sum())}
+        resolved = k not in PHOTON or photons >= MIN_PHOTON_COUNTS
+        e["in_max"] = resolved
+        if resolved:
+            worst = max(worst, e["max_rel"])
+        out[k] = e
+    te_a, te_b = float(ref["Out_tot_E"].sum()), float(mix["Out_tot_E"].sum())
+    return worst, out, abs(te_b - te_a) / abs(te_a), photons
+
+
+def run_leg(tg, abi, args, precision, first_iter):
+    """Tables, warmup and the timed steps on one precision path; tallies of the
+    timed steps accumulate in one set."""
+    lib = tg.load_library()
+    mat, ion, opt, run = tg.config_material(args.config)
+    mat.prepare(lib)
+    opt.gen_capacity = args.gen_capacity
+    opt.precision = precision
+    ctx = tg.Context(mat, ion, opt)
+    t0 = time.perf_counter()
+    ctx.build_tables()
+    t_tables = time.perf_counter() - t0
+    name = "mixed" if precision else "fp64"
+    print(f"[bench] {name}: tables built in {t_tables:.2f} s", file=sys.stderr, flush=True)
+    R, V = tg.radial_grid(run["NR"], run["Rmin"], run["Rmax"], mat.layer)
+    B = args.iters_per_step
+    seed = run["seed"]
+    warm_iter = first_iter + args.steps * B          # warmup draws streams the timed steps do not use
+    for w in range(args.warmup):
+        ctx.run_mc(B, seed, R, V, first_iter=warm_iter + w * B)
+        print(f"[bench] {name}: warmup step {w} done", file=sys.stderr, flush=True)
+    ctx.set_profiling(True)
+    tal = ctx.new_tallies(len(R))
+    ev = np.zeros(8, dtype=np.int64)
+    t_total, host_ms = 0.0, []
+    for k in range(args.steps):
+        before = tal.n_events.copy()
+        ts = time.perf_counter()
+        ctx.run_mc(B, seed, R, V, tallies=tal, first_iter=first_iter + k * B)
+        t_total += time.perf_counter() - ts
+        step_ev = tal.n_events - before
+        ev += step_ev
+        print(f"[bench] {name}: step {k}: {int(step_ev.sum())} events", file=sys.stderr, flush=True)
+        tr_s, tot_s, _ = ctx.timing()
+        host_ms.append(1e3 * (tot_s - tr_s))
+    ms_tr, n_tr = ctx.kernel_time(abi.K_TRANSPORT)
+    ms_shi, n_shi = ctx.kernel_time(abi.K_SHI)
+    ctx.set_profiling(False)
+    # live particles at the time nodes (snapshot terms of the byte model)
+    snaps = [float(tal["Out_tot_Ne"].sum()), float((tal["Out_nh"] / V[None, :, None, None]).sum()),
+             float(tal["Out_tot_Nphot"].sum())]
+    return {"ctx": ctx, "mat": mat, "ion": ion, "opt": opt, "run": run, "R": R, "V": V, "tal": tal,
+            "ev": ev, "t_total": t_total, "host_ms": host_ms, "ms_tr": ms_tr, "n_tr": n_tr,
+            "ms_shi": ms_shi, "n_shi": n_shi, "snaps": snaps, "t_tables": t_tables}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--iters-per-step", type=int, default=4096)
+    ap.add_argument("--precision", choices=["mixed", "fp64"], default="mixed",
+                    help="path whose events/s is the value (mixed: also runs the FP64 path for the comparison)")
+    ap.add_argument("--no-compare", action="store_true",
+                    help="mixed only: skip the FP64 leg (no speed-up or tally error reported)")
     ap.add_argument("--cpu-iters", type=int, default=2,
                     help="oracle iterations per host thread for the CPU baseline (0 = skip)")
     ap.add_argument("--config", default="config2")
@@ -84,58 +175,40 @@ def main():
 
     import trekis_gpu as tg
     from trekis_gpu import abi
-    lib = tg.load_library()
-    mat, ion, opt, run = tg.config_material(args.config)
-    mat.prepare(lib)
-    opt.gen_capacity = args.gen_capacity
-    ctx = tg.Context(mat, ion, opt)
-    t0 = time.perf_counter()
-    ctx.build_tables()
-    t_tables = time.perf_counter() - t0
-    print(f"[bench] tables built in {t_tables:.2f} s", file=sys.stderr, flush=True)
-    R, V = tg.radial_grid(run["NR"], run["Rmin"], run["Rmax"], mat.layer)
+    mixed = args.precision == "mixed"
+    main_leg = run_leg(tg, abi, args, 1 if mixed else 0, 0)
+    mat, ion, opt, run = main_leg["mat"], main_leg["ion"], main_leg["opt"], main_leg["run"]
+    R, V = main_leg["R"], main_leg["V"]
     B = args.iters_per_step
-    seed = run["seed"]
-    it = 0
-    for w in range(args.warmup):
-        ctx.run_mc(B, seed, R, V, first_iter=it)
-        it += B
-        print(f"[bench] warmup step {w} done", file=sys.stderr, flush=True)
-
-    ctx.set_profiling(True)
-    ev = np.zeros(8, dtype=np.int64)
-    snaps = [0.0, 0.0, 0.0]
-    t_total = 0.0
-    host_ms = []
-    for k in range(args.steps):
-        t = ctx.new_tallies(len(R))
-        ts = time.perf_counter()
-        ctx.run_mc(B, seed, R, V, tallies=t, first_iter=it)
-        t_total += time.perf_counter() - ts
-        it += B
-        ev += t.n_events
-        snaps[0] += float(t["Out_tot_Ne"].sum())
-        snaps[1] += float((t["Out_nh"] / V[None, :, None, None]).sum())
-        snaps[2] += float(t["Out_tot_Nphot"].sum())
-        print(f"[bench] step {k}: {int(t.n_events.sum())} events", file=sys.stderr, flush=True)
-        tr_s, tot_s, _ = ctx.timing()
-        host_ms.append(1e3 * (tot_s - tr_s))
-    ms_tr, n_tr = ctx.kernel_time(abi.K_TRANSPORT)
-    ms_shi, n_shi = ctx.kernel_time(abi.K_SHI)
-    ctx.set_profiling(False)
-
+    ev, t_total = main_leg["ev"], main_leg["t_total"]
     total_ev = int(ev.sum())
     value = total_ev / t_total
+    ms_tr, n_tr = main_leg["ms_tr"], main_leg["n_tr"]
     nsh = sum(mat.n_shells)
-    b_tr = transport_bytes(ev, nsh, *[int(round(s)) for s in snaps], opt.kind_of_EMFP == 1)
+    b_tr = transport_bytes(ev, nsh, *[int(round(s)) for s in main_leg["snaps"]], opt.kind_of_EMFP == 1)
     per_launch = b_tr / max(n_tr, 1)
     mean_s = (ms_tr / max(n_tr, 1)) * 1e-3
     achieved = per_launch / mean_s / 1e9 if mean_s > 0 else 0.0
 
+    compare = None
+    if mixed and not args.no_compare:
+        ref = run_leg(tg, abi, args, 0, 0)
+        worst, per, de, photons = tally_errors(ref["tal"], main_leg["tal"])
+        ref_value = float(ref["ev"].sum()) / ref["t_total"]
+        compare = {"fp64_value": ref_value, "speedup_vs_fp64": value / ref_value,
+                   "fp64_transport_mean_launch_ms": ref["ms_tr"] / max(ref["n_tr"], 1),
+                   "max_radial_tally_rel_error": worst, "tot_E_rel_diff": de,
+                   "photon_counts_fp64": photons, "tallies": per,
+                   "error_rule": "per radial tally (time x radius): largest relative difference over bins "
+                                 "holding >= 1% of the tally's peak, same iterations and random streams; "
+                                 f"photon tallies enter the maximum only with >= {MIN_PHOTON_COUNTS:g} "
+                                 "photon counts"}
+        ref["ctx"].close()
+
     # measured HBM bytes per launch of the transport kernel (FETCH_SIZE and
     # WRITE_SIZE passes, scaled by a plain copy), for this workload if taken
     traffic = None
-    tf = os.path.join(ROOT, "profiles", f"k_transport__al2o3_au11_{B}iter_traffic.txt")
+    tf = os.path.join(ROOT, "profiles", f"k_transport__al2o3_au11_{B}iter_{args.precision}_traffic.txt")
     if args.config == "config2" and os.path.exists(tf):
         kv = dict(l.strip().split("=", 1) for l in open(tf) if "=" in l and not l.startswith("#"))
         traffic = float(kv["traffic_bytes_per_launch"])
@@ -147,8 +220,9 @@ def main():
         # one iteration on one core
         sys.path.insert(0, os.path.join(ROOT, "oracle"))
         from oracle import Oracle
+        seed = run["seed"]
         orc = Oracle(mat, ion, opt)
-        orc.inject_from(ctx)
+        orc.inject_from(main_leg["ctx"])
         threads = max(1, min(int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1)), 16))
         n_all = args.cpu_iters * threads
         ts = time.perf_counter()
@@ -158,13 +232,15 @@ def main():
         c1 = orc.run_mc(1, seed, R, V, tg.TallySet(orc.Nt, len(R), mat))
         dt1 = time.perf_counter() - ts1
         cpu = {"value": float(c.n_events.sum()) / dt, "unit": "events/s", "cores": threads, "kind": "port",
-               "sample": f"{n_all} iterations of the same workload on the GPU-built tables, OpenMP over "
+               "sample": f"{n_all} iterations of the same workload on the GPU-built tables, FP64, OpenMP over "
                          f"iterations on {threads} threads: {int(c.n_events.sum())} events in {dt:.1f} s; "
                          f"one iteration on one core: {int(c1.n_events.sum())} events in {dt1:.1f} s",
                "single_core_value": float(c1.n_events.sum()) / dt1}
 
+    kname = "trk::mx::k_transport" if mixed else "trk::fp64::k_transport"
     line = {
-        "metric": "collision events/s on 1 GPU (FP64 path)",
+        "metric": "collision events/s on 1 GPU, mixed vs FP64, at <=1% radial-tally error" if mixed
+                  else "collision events/s on 1 GPU (FP64 path)",
         "value": value,
         "unit": "events/s",
         "n_gpus": 1,
@@ -174,16 +250,18 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": "f32 sampling / f64 state and tallies" if mixed else "f64",
         "data": "synthetic",
         "config": {"workload": f"{mat.name} target, Au 11 MeV/u ion, 0-1000 fs x 100 radial bins, "
-                               f"{B} iterations per step", "iters_per_step": B, "events": total_ev,
-                   "events_by_kind": dict(zip(abi.EVENT_NAMES, ev.tolist())), "table_build_s": t_tables,
-                   "ion_init_kernel_ms": ms_shi, "ion_init_launches": n_shi,
-                   "outside_batches_ms_per_step": sum(host_ms) / len(host_ms)},
+                               f"{B} iterations per step", "precision": args.precision, "iters_per_step": B,
+                   "events": total_ev, "events_by_kind": dict(zip(abi.EVENT_NAMES, ev.tolist())),
+                   "table_build_s": main_leg["t_tables"], "ion_init_kernel_ms": main_leg["ms_shi"],
+                   "ion_init_launches": main_leg["n_shi"],
+                   "outside_batches_ms_per_step": sum(main_leg["host_ms"]) / len(main_leg["host_ms"]),
+                   "vs_fp64": compare},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "kernel": "k_transport", "launches": n_tr, "mean_launch_ms": ms_tr / max(n_tr, 1),
+                     "kernel": kname, "launches": n_tr, "mean_launch_ms": ms_tr / max(n_tr, 1),
                      "alg_bytes_per_launch": per_launch,
                      "traffic_GBs": (traffic / mean_s / 1e9) if (traffic and mean_s > 0) else None},
         "cpu_baseline": cpu,

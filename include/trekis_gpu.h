@@ -119,7 +119,9 @@ typedef struct trk_options {
                                            <= 0 selects the reference value 175.6e6*2/1836 (AI:143) */
     double  shi_grid_Emax;              /* [eV] top of the ion energy grid;
                                            <= 0 selects the reference value 175.6e6/2*M (AI:2272) */
-    int32_t precision;                  /* 0 FP64 everywhere, 1 mixed (FP32 state, FP64 tallies) */
+    int32_t precision;                  /* 0 FP64 everywhere (replays the oracle bit for bit);
+                                           1 mixed: FP32 elementary functions in the event code,
+                                           FP64 state, tables, arithmetic and tallies */
     int32_t batch;                      /* iterations per device batch, 0 = automatic */
     int32_t block;                      /* threads per workgroup for transport, 0 = automatic */
     int32_t tally_replicas;             /* copies of the tally arrays (contention), 0 = automatic */

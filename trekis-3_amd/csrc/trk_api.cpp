@@ -57,7 +57,7 @@ static int validate(const trk_material *m, const trk_ion *ion, const trk_options
     if (!(o->Tim > 0.0) || !(o->dt > 0.0)) { why = "Tim and dt must be positive"; return 1; }
     if (o->dt_flag >= 1 && o->dt < 2.0) { why = "a logarithmic time grid needs dt >= 2 (RF:299)"; return 1; }
     if (o->kind_of_DR < 1 || o->kind_of_DR > 3) { why = "kind_of_DR must be 1, 2 or 3"; return 1; }
-    if (o->precision != 0) { why = "only precision 0 (FP64) is built"; return 1; }
+    if (o->precision != 0 && o->precision != 1) { why = "precision must be 0 (FP64) or 1 (mixed)"; return 1; }
     return 0;
 }
 

@@ -103,291 +103,32 @@ struct McDev {
 
 enum { EV_SHI = 0, EV_E_INEL, EV_E_EL, EV_H_INEL, EV_H_EL, EV_AUGER, EV_RAD, EV_PHOT };
 
-struct PState {
-    double E, t0, tn, X, Y, Z, L, th, ph, Ehkin, Mass;
-    int sh;
-    uint64_t id;
-    uint32_t nd, ne, iter;
-    // electrons: total inelastic and elastic mean free paths at energy cE and
-    // the elastic-grid bracket of cE, kept from the previous event's free-path
-    // draw (the next event looks them up again at the same energy)
-    double cE, cIM, cEM;
-    int cN;
-    // the particle's last Philox block (index + 1, 0 = none): draws 2k and
-    // 2k+1 come from the same block, so every other draw is a lookup
-    uint32_t cb, cw0, cw1, cw2, cw3;
-};
+// d.overflow bits: list capacities, the per-particle event limit, NaN state.
+constexpr int kFlagPairs = 1, kFlagPhotons = 2, kFlagEvents = 4, kFlagNaN = 8;
+// Events one particle may process in one launch.  A history needs a few
+// thousand at most; the limit only guarantees that every wave ends.
+constexpr uint32_t kMaxEvents = 1u << 22;
 
-// One Philox block of a particle's stream.
-__device__ __forceinline__ void philox_block(uint32_t blk, uint64_t id, uint32_t iter, uint64_t seed, uint32_t *w)
-{
-    trk_u32x4 c;
-    c.v[0] = blk; c.v[1] = (uint32_t)id; c.v[2] = (uint32_t)(id >> 32); c.v[3] = iter;
-    trk_u32x4 r = trk_philox4x32_10(c, (uint32_t)seed, (uint32_t)(seed >> 32));
-    w[0] = r.v[0]; w[1] = r.v[1]; w[2] = r.v[2]; w[3] = r.v[3];
-}
+enum { K_EL = 0, K_HO = 1, K_PH = 2, K_ION = 3 };
 
-// Draw n of the particle's stream without consuming it; equals
-// trk_uniform(seed, id, iter, n) bit for bit.
-TRK_DEV double rn_at(const McDev &d, PState &p, uint32_t n)
-{
-    const uint32_t blk = n >> 1;
-    if (p.cb != blk + 1u) {
-        uint32_t w[4];
-        philox_block(blk, p.id, p.iter, d.seed, w);
-        p.cw0 = w[0]; p.cw1 = w[1]; p.cw2 = w[2]; p.cw3 = w[3];
-        p.cb = blk + 1u;
-    }
-    const uint32_t a = (n & 1u) ? p.cw2 : p.cw0;
-    const uint32_t b = (n & 1u) ? p.cw3 : p.cw1;
-    const uint64_t u = ((((uint64_t)a) << 32) | (uint64_t)b) >> 11;
-    return ((double)u + 0.5) * 0x1.0p-53;
-}
+enum { C_E_EL = 0, C_E_INEL, C_H_EL, C_H_INEL, C_DEEP, C_PHOT, C_ION, C_COUNT };
 
-TRK_DEV double rn(const McDev &d, PState &p) { return rn_at(d, p, p.nd++); }
+// A wave runs a category of events only when at least this many of its lanes
+// want it (or when it is the most wanted one); the other lanes wait with
+// their state untouched.  Rare, long branches (ionisation, decays) then run
+// for many lanes at once instead of stalling the wave at almost every step.
+constexpr int kBatchLanes = 16;
+constexpr int kMaxWait = 8;
 
-TRK_DEV void load(const PList &l, int i, PState &p, int hole)
-{
-    p.E = l.E[i]; p.t0 = l.t0[i]; p.tn = l.tn[i]; p.X = l.X[i]; p.Y = l.Y[i]; p.Z = l.Z[i];
-    p.L = l.L[i]; p.th = l.th[i]; p.ph = l.ph[i];
-    if (hole) { p.Ehkin = l.Ehkin[i]; p.Mass = l.Mass[i]; p.sh = l.sh[i]; }
-    else { p.Ehkin = 0.0; p.Mass = 1.0; p.sh = -1; }
-    p.id = l.id[i]; p.nd = l.nd[i]; p.ne = l.ne[i]; p.iter = l.iter[i];
-    p.cE = TRK_NAN; p.cIM = 0.0; p.cEM = 0.0; p.cN = 0;
-    p.cb = 0u;
-}
+// Events one particle may take in one round.  A particle with more left is
+// written to the next round's list; this bounds the tail of every launch by
+// the cost of kRoundEvents events instead of the longest history.
+constexpr uint32_t kRoundEvents = 512;
 
-TRK_DEV void store(const PList &l, int i, const PState &p, int hole)
-{
-    l.E[i] = p.E; l.t0[i] = p.t0; l.tn[i] = p.tn; l.X[i] = p.X; l.Y[i] = p.Y; l.Z[i] = p.Z;
-    l.L[i] = p.L; l.th[i] = p.th; l.ph[i] = p.ph;
-    if (hole) { l.Ehkin[i] = p.Ehkin; l.Mass[i] = p.Mass; l.sh[i] = p.sh; }
-    l.id[i] = p.id; l.nd[i] = p.nd; l.ne[i] = p.ne; l.iter[i] = p.iter;
-}
+extern __shared__ double lds_tab[];
 
-// fresh slot values of the reference arrays (MC:1948-1951, Objects.f90:471-474)
-TRK_DEV void init_child(PState &c, uint64_t parent, uint32_t ev, uint32_t slot, uint32_t iter, int hole)
-{
-    c.E = 0.0; c.t0 = 0.0; c.X = 0.0; c.Y = 0.0; c.Z = 0.0; c.L = 0.0; c.th = 0.0; c.ph = 0.0;
-    if (hole) { c.tn = 1.0e21; c.L = 1.0e30; c.Mass = 1.0e30; c.Ehkin = 0.0; c.sh = -1; }
-    else { c.tn = 1.0e20; c.Mass = 1.0; c.Ehkin = 0.0; c.sh = -1; }
-    c.id = trk_child_id(parent, ev, slot);
-    c.nd = 0; c.ne = 0; c.iter = iter;
-    c.cb = 0u;
-}
+constexpr int kTransportBlock = 768;   // 12 waves: one workgroup per CU at 3 waves per SIMD
 
-// ---------------------------------------------------------------- tallies
-TRK_DEV void fx_add(unsigned long long *p, double v)
-{
-    long long x = (long long)__builtin_rint(v * 0x1.0p+40);
-    unsigned long long lo = (unsigned long long)x & 0xFFFFFFFFull;
-    long long hi = x >> 32;
-    if (lo) atomicAdd(p, lo);
-    if (hi) atomicAdd(p + 1, (unsigned long long)hi);
-}
-
-TRK_DEV void cnt_add(unsigned long long *p, long long v) { atomicAdd(p, (unsigned long long)v); }
-
-// number of time nodes with tim_glob <= t
-TRK_DEV int nodes_le(const McDev &d, double t)
-{
-    if (d.inv_dt > 0.0) {
-        // linear grid: start next to t/dt and step to the exact count
-        if (!(t >= d.tim_glob[0])) return 0;
-        if (t >= d.tim_glob[d.Nt - 1]) return d.Nt;
-        int i = (int)(t * d.inv_dt);
-        if (i > d.Nt) i = d.Nt;
-        if (i < 0) i = 0;
-        while (i > 0 && d.tim_glob[i - 1] > t) --i;
-        while (i < d.Nt && d.tim_glob[i] <= t) ++i;
-        return i;
-    }
-    int lo = 0, hi = d.Nt;
-    while (lo < hi) {
-        int mid = (lo + hi) >> 1;
-        if (d.tim_glob[mid] <= t) lo = mid + 1;
-        else hi = mid;
-    }
-    return lo;
-}
-
-// Range add over 0-based node rows [r0, r1] of one particle kind in radial bin j.
-TRK_DEV void range_e(const McDev &d, int rep, int r0, int r1, int j, double E)
-{
-    const Tally &T = d.T;
-    size_t base = ((size_t)rep * T.Nt) * T.NR;
-    size_t a = base + (size_t)r0 * T.NR + j;
-    cnt_add(T.cnt_e + a, 1);
-    fx_add(T.E_e + 2 * a, E);
-    if (r1 + 1 < T.Nt) {
-        size_t b = base + (size_t)(r1 + 1) * T.NR + j;
-        cnt_add(T.cnt_e + b, -1);
-        fx_add(T.E_e + 2 * b, -E);
-    }
-}
-
-TRK_DEV void range_p(const McDev &d, int rep, int r0, int r1, int j, double E)
-{
-    const Tally &T = d.T;
-    size_t base = ((size_t)rep * T.Nt) * T.NR;
-    size_t a = base + (size_t)r0 * T.NR + j;
-    cnt_add(T.cnt_p + a, 1);
-    fx_add(T.E_p + 2 * a, E);
-    if (r1 + 1 < T.Nt) {
-        size_t b = base + (size_t)(r1 + 1) * T.NR + j;
-        cnt_add(T.cnt_p + b, -1);
-        fx_add(T.E_p + 2 * b, -E);
-    }
-}
-
-TRK_DEV void range_h(const McDev &d, int rep, int r0, int r1, int j, int s, double E, double Ek)
-{
-    const Tally &T = d.T;
-    size_t base = ((size_t)rep * T.Nt) * T.NR;
-    size_t a = (base + (size_t)r0 * T.NR + j) * T.nsh + s;
-    cnt_add(T.cnt_h + a, 1);
-    fx_add(T.E_h + 2 * a, E);
-    fx_add(T.Ek_h + 2 * a, Ek);
-    if (r1 + 1 < T.Nt) {
-        size_t b = (base + (size_t)(r1 + 1) * T.NR + j) * T.nsh + s;
-        cnt_add(T.cnt_h + b, -1);
-        fx_add(T.E_h + 2 * b, -E);
-        fx_add(T.Ek_h + 2 * b, -Ek);
-    }
-}
-
-// Snapshot contribution of one state, valid from its creation time t0 until
-// its next event tn (MC:998-1097): nodes i with t0 < tim_glob(i) <= tn.
-// kind 0 electron, 1 hole, 2 photon.
-TRK_DEV double vel_hole(const PState &h);
-
-TRK_DEV void range_i(int *a, int stride, int Nt, int r0, int r1)
-{
-    atomicAdd(a + (size_t)r0 * stride, 1);
-    if (r1 + 1 < Nt) atomicAdd(a + (size_t)(r1 + 1) * stride, -1);
-}
-
-// bin of a polar angle on the 1-degree grid 1..180 (find_mono_1d on it)
-TRK_DEV int theta_bin(double th)
-{
-    double xx = (th / kPi) * 180.0;
-    if (!(xx >= 1.0)) return 0;
-    if (xx >= 180.0) return 179;
-    int b = (int)xx;              // number of grid values <= xx
-    return b < 179 ? b : 179;
-}
-
-// Calculated_statistics, MC:1022-1110: the per-iteration normalised tallies
-TRK_DEV void spectra(const McDev &d, const PState &p, int kind, int r0, int r1)
-{
-    const Spec &S = d.S;
-    const int Nt = d.Nt;
-    const size_t b = (size_t)(p.iter - (uint32_t)S.iter0);
-    if (kind == 0) {
-        range_i(S.nel + b * Nt, 1, Nt, r0, r1);
-        int j = find_mono_1d(d.R, d.NR, p.E) - 1;
-        range_i(S.eve + (b * Nt) * d.NR + j, d.NR, Nt, r0, r1);
-        if (p.E > 0.0) {
-            double th0 = (p.E > d.cutoff_eff) ? p.th : 0.0;
-            range_i(S.th + (b * Nt) * 180 + theta_bin(th0), 180, Nt, r0, r1);
-        }
-    } else if (kind == 1) {
-        if (p.sh == d.m.vb) {
-            range_i(S.nvb + b * Nt, 1, Nt, r0, r1);
-            if (S.ND > 1) {
-                int jj = find_mono_1d(d.m.dosE, d.m.n_dos, p.Ehkin) - 1;
-                range_i(S.ehve + (b * Nt) * S.ND + jj, S.ND, Nt, r0, r1);
-            }
-        }
-        if (p.Mass < 1.0e3 && p.Ehkin > d.cutoff_eff) range_i(S.thh + (b * Nt) * 180 + theta_bin(p.th), 180, Nt, r0, r1);
-        if (p.Mass < 1.0e3 && p.L < 1.0e3) {
-            range_i(S.dcn + b * Nt, 1, Nt, r0, r1);
-            double v = (((1.0 / 3.0) * vel_hole(p)) * p.L) * 1.0e-6;
-            fx_add(S.dcs + 2 * (b * Nt + r0), v);
-            if (r1 + 1 < Nt) fx_add(S.dcs + 2 * (b * Nt + r1 + 1), -v);
-        }
-    }
-}
-
-TRK_DEV void segment(const McDev &d, const PState &p, int kind, int rep)
-{
-    int r0 = nodes_le(d, p.t0);                         // first node row with tim > t0
-    int r1 = (p.tn < d.T_last) ? nodes_le(d, p.tn) - 1 : d.Nt - 1;
-    if (r0 > r1) return;
-    if (d.S.on) spectra(d, p, kind, r0, r1);
-    double V = 0.0, st = 0.0, sp = 0.0, cp = 0.0;
-    bool moving;
-    if (kind == 0) {
-        moving = p.E > d.cutoff_eff;
-        if (moving) V = trk_sqrt(((2.0 * p.E) * kE) / kMe);
-    } else if (kind == 1) {
-        moving = (p.Mass < 1.0e3) && (p.Ehkin > d.cutoff_eff);
-        if (moving) {
-            if (p.Mass < 1.0e6 && !((p.Ehkin < -1.0e-6) || (p.Mass < 1.0e-10)) && !(trk_fabs(p.Ehkin) < 1.0e-6))
-                V = trk_sqrt(((2.0 * p.Ehkin) * kE) / (p.Mass * kMe));
-            else
-                V = 0.0;
-        }
-    } else {
-        if (!(p.E > 0.0)) return;
-        moving = true;
-        V = kC;
-    }
-    if (moving) { st = trk_sin(p.th); trk_sincos(p.ph, &sp, &cp); }
-    else { st = trk_sin(0.0); trk_sincos(0.0, &sp, &cp); }
-    int jprev = -1, rstart = r0;
-    for (int r = r0; r <= r1; ++r) {
-        double L0 = 0.0;
-        if (moving) {
-            L0 = ((V * (d.tim_glob[r] - p.t0)) * 1.0e-5);
-            if (L0 < 0.0) L0 = 0.0;
-        }
-        double X = p.X + (L0 * st) * sp;
-        double Y = p.Y + (L0 * st) * cp;
-        double R = trk_sqrt(X * X + Y * Y);
-        int j = find_mono_1d(d.R, d.NR, R) - 1;
-        if (j < 0) j = 0;
-        if (jprev >= 0 && j != jprev) {
-            if (kind == 0) range_e(d, rep, rstart, r - 1, jprev, p.E);
-            else if (kind == 1) range_h(d, rep, rstart, r - 1, jprev, p.sh, p.E, p.Ehkin);
-            else range_p(d, rep, rstart, r - 1, jprev, p.E);
-            rstart = r;
-        }
-        jprev = j;
-        if (!moving) { r = r1; }
-    }
-    if (kind == 0) range_e(d, rep, rstart, r1, jprev, p.E);
-    else if (kind == 1) range_h(d, rep, rstart, r1, jprev, p.sh, p.E, p.Ehkin);
-    else range_p(d, rep, rstart, r1, jprev, p.E);
-}
-
-// Out_Elat: per elastic event in the grid_do pass of its time node (MC:2436, 2719)
-TRK_DEV void tally_elat(const McDev &d, int rep, double t_ev, double X, double Y, double dE)
-{
-    int r = nodes_le(d, t_ev);
-    if (r >= d.Nt) return;
-    double R = trk_sqrt(X * X + Y * Y);
-    int j = find_mono_1d(d.R, d.NR, R) - 1;
-    if (j < 0) j = 0;
-    const Tally &T = d.T;
-    size_t a = (((size_t)rep * T.Nt) + r) * T.NR + j;
-    fx_add(T.Elat + 2 * a, dE);
-}
-
-TRK_DEV void trace_ev(const McDev &d, int kind, const PState &p, uint32_t ev, double t_ev,
-                      double E, double E2, double X, double Y, double Z)
-{
-    if (!d.trace) return;
-    unsigned long long k = atomicAdd(d.trace_n, 1ull);
-    if ((long long)k >= d.trace_cap) return;
-    double *r = d.trace + 12 * k;
-    r[0] = (double)p.iter;
-    r[1] = __longlong_as_double((long long)p.id);
-    r[2] = (double)ev; r[3] = (double)kind; r[4] = t_ev; r[5] = E; r[6] = E2;
-    r[7] = X; r[8] = Y; r[9] = Z; r[10] = p.tn; r[11] = (double)p.nd;
-}
-
-// ---------------------------------------------------------------- lookups
 // Next_free_path_2d (MC:1874-1899) and _1d (MC:1835-1871)
 TRK_HD double nfp_2d(const double *Ea, const double *La, int n, double E)
 {
@@ -402,1033 +143,18 @@ TRK_HD double nfp_2d(const double *Ea, const double *La, int n, double E)
     return interpolate(5, Ea[Nl - 1], Ea[N - 1], La[Nl - 1], La[N - 1], E);
 }
 
-TRK_HD double nfp_1d(const double *Ea, const double *La, int n, double E)
-{
-    int N = find_mono_1d(Ea, n, E);
-    if (N < 0) return 1.0e30;
-    if (N == 1) {
-        double M = interpolate(1, Ea[0], Ea[1], La[0], La[1], E);
-        return (M < Ea[0]) ? Ea[0] : M;
-    }
-    int Nl = N - 1;
-    if (La[Nl - 1] >= 1.0e16) return La[Nl - 1];
-    return interpolate(5, Ea[Nl - 1], Ea[N - 1], La[Nl - 1], La[N - 1], E);
-}
+// The event code, twice: deterministic FP64 (fp64::) and mixed precision (mx::).
+namespace fp64 {
+#define TRK_MX 0
+#include "trk_events.inc"
+#undef TRK_MX
+}  // namespace fp64
 
-// Interpolate flag 5 with the logs of the nodes, the values and the argument
-// given: the same operations on the same numbers as interpolate(5, ...).
-TRK_HD double interp5_ln(double E1, double E2, double S1, double S2, double E, double lE1, double lE2,
-                         double lS1, double lS2, double lE)
-{
-    if (trk_fabs(E2 - E1) < 1.0e-6) return (S1 > S2) ? S1 : S2;
-    if (E == E1) return S1;
-    double o = lS1 + ((lS2 - lS1) / (lE2 - lE1)) * (lE - lE1);
-    return trk_exp(o);
-}
-
-// nfp_2d / nfp_1d on a table with precomputed logs (lEa, lLa); lE = trk_log(E).
-// *Nout receives the bracket index of the search.
-TRK_HD double nfp_2d_ln(const double *Ea, const double *La, const double *lEa, const double *lLa, int n, double E,
-                        double lE, int *Nout)
-{
-    int N = find_mono_2d(Ea, n, E);
-    *Nout = N;
-    if (N == 1) {
-        double M = interpolate(1, Ea[0], Ea[1], La[0], La[1], E);
-        return (M < La[0]) ? La[0] : M;
-    }
-    int Nl = N - 1;
-    if (La[Nl - 1] >= 1.0e16) return La[Nl - 1];
-    return interp5_ln(Ea[Nl - 1], Ea[N - 1], La[Nl - 1], La[N - 1], E, lEa[Nl - 1], lEa[N - 1], lLa[Nl - 1],
-                      lLa[N - 1], lE);
-}
-
-TRK_HD double nfp_1d_ln(const double *Ea, const double *La, const double *lEa, const double *lLa, int n, double E,
-                        double lE)
-{
-    int N = find_mono_1d(Ea, n, E);
-    if (N == 1) {
-        double M = interpolate(1, Ea[0], Ea[1], La[0], La[1], E);
-        return (M < Ea[0]) ? Ea[0] : M;
-    }
-    int Nl = N - 1;
-    if (La[Nl - 1] >= 1.0e16) return La[Nl - 1];
-    return interp5_ln(Ea[Nl - 1], Ea[N - 1], La[Nl - 1], La[N - 1], E, lEa[Nl - 1], lEa[N - 1], lLa[Nl - 1],
-                      lLa[N - 1], lE);
-}
-
-// Which_shell, MC:1786-1832 (per-shell lambda on a shared grid)
-TRK_DEV double shell_mfp(const double *Eg, const double *Lsh, int n, int N, int s, double E)
-{
-    if (N == 1) return 1.0e20;
-    const double *L = Lsh + (size_t)s * n;
-    if ((L[N - 2] == L[N - 1]) || (L[N - 2] > 1.0e20)) return L[N - 2];
-    return interpolate(5, Eg[N - 2], Eg[N - 1], L[N - 2], L[N - 1], E);
-}
-
-TRK_DEV int which_shell(const McDev &d, const double *Eg, const double *Lsh, int n, double E, PState &p)
-{
-    const int nsh = d.m.nsh;
-    int N = find_mono_1d(Eg, n, E);
-    if (N < 0) N = 1;
-    double tot = 0.0;
-    if (nsh <= 8) {
-        double tmp[8];
-#pragma unroll
-        for (int s = 0; s < 8; ++s) {
-            tmp[s] = 0.0;
-            if (s < nsh) {
-                tmp[s] = 1.0 / shell_mfp(Eg, Lsh, n, N, s, E);
-                tot = tot + tmp[s];
-            }
-        }
-        tot = rn(d, p) * tot;
-        double sum = 0.0;
-#pragma unroll
-        for (int s = 0; s < 8; ++s) {
-            if (s < nsh) {
-                sum = sum + tmp[s];
-                if (sum >= tot) return s;
-            }
-        }
-        return nsh - 1;
-    }
-    for (int s = 0; s < nsh; ++s) tot = tot + 1.0 / shell_mfp(Eg, Lsh, n, N, s, E);
-    tot = rn(d, p) * tot;
-    double sum = 0.0;
-    for (int s = 0; s < nsh; ++s) {
-        sum = sum + 1.0 / shell_mfp(Eg, Lsh, n, N, s, E);
-        if (sum >= tot) return s;
-    }
-    return nsh - 1;
-}
-
-// interpolate_transferred_energy, Cross_sections.f90:1968-2045
-TRK_DEV double interp_transfer(const double *Eg, int n, const int64_t *roff, const double *hwa,
-                               const double *lama, int shell, double Ele, double L_need)
-{
-    int iE = find_mono_1d(Eg, n, Ele);
-    if (iE < 0) return TRK_NAN;
-    if (iE > 1 && trk_fabs(Eg[iE - 2] - Ele) < 1.0e-6) iE = iE - 1;
-    size_t slot = (size_t)shell * n + (iE - 1);
-    int64_t o = roff[slot];
-    int len = (int)(roff[slot + 1] - o);
-    if (len < 1) return TRK_NAN;
-    const double *hw = hwa + o, *lam = lama + o;
-    int ih = find_decr(lam, len, L_need);
-    if (ih < 0) return TRK_NAN;
-    double hw1 = ((ih == 1) || (ih == len)) ? hw[ih - 1] : interpolate(5, lam[ih - 1], lam[ih], hw[ih - 1], hw[ih], L_need);
-    double out = hw1;
-    if (iE > 1) {
-        iE = iE - 1;
-        slot = (size_t)shell * n + (iE - 1);
-        o = roff[slot];
-        len = (int)(roff[slot + 1] - o);
-        if (len < 1) return TRK_NAN;
-        hw = hwa + o;
-        lam = lama + o;
-        ih = find_decr(lam, len, L_need);
-        double hw2 = ((ih == 1) || (ih == len)) ? hw[ih - 1] : interpolate(5, lam[ih - 1], lam[ih], hw[ih - 1], hw[ih], L_need);
-        if ((hw1 < 1.0e-10) || (hw2 < 1.0e-10)) out = interpolate(1, Eg[iE - 1], Eg[iE], hw1, hw2, Ele);
-        else out = interpolate(5, Eg[iE - 1], Eg[iE], hw1, hw2, Ele);
-    }
-    return out;
-}
-
-TRK_DEV double hole_mass_tab(const McDev &d, double Ele)
-{
-    if (d.m.hole_mass >= 0.0) return d.m.hole_mass;
-    return d.m.dosM[find_mono_1d(d.m.dosE, d.m.n_dos, Ele) - 1];
-}
-
-// Electron_energy_transfer_inelastic, Cross_sections.f90:1793-1871
-TRK_DEV double transfer_inel(const McDev &d, double Ele, int shell, double L_tot, int hole, PState &p)
-{
-    double RN = rn(d, p);
-    double L_need = L_tot / RN;
-    double Emin = d.m.Ip[shell];
-    double Egap = d.m.Egap;
-    if (Emin <= 1.0e-3) Emin = 1.0e-3;
-    double Emax, Mass;
-    if (!hole) {
-        Emax = (Ele + Emin) / 2.0;
-    } else {
-        Mass = hole_mass_tab(d, Ele);
-        Emax = ((4.0 * Ele) * Mass) / ((Mass + 1.0) * (Mass + 1.0));
-    }
-    if (d.m.plasmon_Emax && Emin == Egap) {
-        double Ep = trk_sqrt((((((d.m.N_VB_el * d.m.At_Dens) * 1.0e6) * kH) * kH) / (kMe * kE0)) + (Egap * Egap));
-        if (Ep >= Emax) Emax = Ep;
-        if (Ele < Emax) Emax = Ele;
-    }
-    double E;
-    if (!hole) E = interp_transfer(d.E, d.nE, d.roff, d.rhw, d.rlam, shell, Ele, L_need);
-    else E = interp_transfer(d.EH, d.nH, d.roff_h, d.rhw_h, d.rlam_h, d.m.vb, Ele, L_need);
-    if (E < Emin) E = Emin;
-    if (E > Emax) E = Emax;
-    if (E != E) E = Emin;
-    return E;
-}
-
-// Electron_energy_transfer_elastic, Cross_sections.f90:2285-2428 (tabulated)
-TRK_DEV double transfer_el(const McDev &d, double Ele, double L_tot, int hole, PState &p)
-{
-    double RN = rn(d, p);
-    double L_need = L_tot / RN;
-    double hw = hole ? interp_transfer(d.EHel, d.nHel, d.roff_hel, d.rhw_hel, d.rlam_hel, 0, Ele, L_need)
-                     : interp_transfer(d.Eel, d.nEel, d.roff_el, d.rhw_el, d.rlam_el, 0, Ele, L_need);
-    if (hw >= Ele) hw = Ele;
-    return hw;
-}
-
-TRK_DEV double rest_energy(double M0) { return ((M0 * kC) * kC) / kE; }
-
-// NRG_transfer_elastic_atomic with Mott_sample_mu, screening_parameter and
-// transfered_E_from_theta, Cross_sections.f90:3517-3613, for every atom kind
-// of the target weighted by its share (MC:2395-2406).  The terms that depend
-// only on the electron energy are formed once, the per-atom constants Z^(2/3)
-// and the atom's rest energy come precomputed; the arithmetic of each term is
-// unchanged.
-TRK_DEV double mott_transfer_all(const McDev &d, double Ee, double M_eff, PState &p)
-{
-    const double Erest = rest_energy(kMe);
-    const double fact = Ee / Erest + 1.0;
-    const double v = kC * trk_sqrt(1.0 - 1.0 / (fact * fact));
-    const double beta = v / kC;
-    const double beta2 = beta * beta;
-    const double tau = Ee / Erest;
-    const double A1 = 1.0 - beta2;
-    const double A2 = ((3.76 * kAlpha) * kAlpha) / beta2;
-    const double S = trk_sqrt(tau / (1.0 + tau));
-    const double mc2 = rest_energy(kMe * M_eff);
-    const double Emc = Ee + mc2;
-    const double E2mc = Ee + 2.0 * mc2;
-    double dE = 0.0;
-    for (int a = 0; a < d.m.n_atoms; ++a) {
-        const double Z = d.m.mott_Z[a];
-        const double Mct2 = d.m.mott_Mct2[a];
-        double RN = rn(d, p);
-        double theta;
-        if (v < 1.0e-6) {
-            theta = 0.0;
-        } else {
-            double nu = (((1.7e-5 * d.m.mott_Z23[a]) * A1) / beta2) * (1.13 + ((A2 * Z) * Z) * S);
-            double mu = (RN * ((2.0 * nu) + 1.0) - nu) / (RN + nu);
-            theta = trk_acos(mu);
-        }
-        double ct = trk_cos(theta);
-        double ct2 = ct * ct;
-        double st2 = 1.0 - ct2;
-        double EmcMc = Emc + Mct2;
-        double W1 = ((Emc * st2) + Mct2) - (ct * trk_sqrt((Mct2 * Mct2) - ((mc2 * mc2) * st2)));
-        double W2 = (Ee * E2mc) / ((EmcMc * EmcMc) - ((Ee * E2mc) * ct2));
-        dE = dE + (W1 * W2) * d.m.mott_Pers[a];
-    }
-    return dE / d.m.sumPers;
-}
-
-TRK_DEV double cos_theta_W(const McDev &d, double E, double W, double M_in, PState &p)
-{
-    double Ein = rest_energy(M_in);
-    double Et = rest_energy(d.m.Mtarget);
-    double E2mc = E + 2.0 * Ein;
-    double EmW = E - W;
-    double W1 = (E * E2mc) - (W * ((E + Ein) + Et));
-    double W2 = ((E * E2mc) * EmW) * (E2mc - W);
-    double mu = (W2 > 0.0) ? W1 / trk_sqrt(W2) : 0.0;
-    if (trk_fabs(mu) > 1.0) mu = trk_cos(kPi * rn(d, p));
-    return mu;
-}
-
-// New_Angles_both, MC:1328-1360
-// ct0 = trk_cos(theta0), computed by the caller for the flight before.
-TRK_DEV void new_angles(double phi0, double theta0, double ct0, double theta, double psi, double &phi1, double &theta1)
-{
-    double sps, cps;
-    trk_sincos(psi, &sps, &cps);
-    double ph = phi0 + ((theta * ct0) * sps);
-    double t = theta0 + (theta * cps);
-    while (t < 0.0) { t = trk_fabs(t); ph = ph + kPi; }
-    while (t > kPi) { t = (2.0 * kPi) - t; ph = ph - kPi; }
-    if (ph > 2.0 * kPi) ph = ph - (((double)floor(ph / (2.0 * kPi))) * 2.0) * kPi;
-    if (ph < 0.0) ph = ph + (((double)ceil(trk_fabs(ph) / (2.0 * kPi))) * 2.0) * kPi;
-    phi1 = ph;
-    theta1 = t;
-}
-
-// integrated-DOS level sampling shared by Electron_recieves_E (MC:1653-1716)
-// and From_where_in_VB (MC:1522-1573)
-TRK_DEV double dos_level(const McDev &d, double Sum, PState &p)
-{
-    const double *E = d.m.dosE, *I = d.m.dosI;
-    const int N = d.m.n_dos;
-    double RN = rn(d, p);
-    double Tot = RN * Sum;
-    int Nt = find_mono_1d(I, N, Tot);
-    if (Nt > 1) return E[Nt - 2] + ((E[Nt - 1] - E[Nt - 2]) * (Tot - I[Nt - 2])) / (I[Nt - 1] - I[Nt - 2]);
-    return E[Nt - 1];
-}
-
-TRK_DEV double receives_E(const McDev &d, double dE, int shell, PState &p)
-{
-    double E = dE - d.m.Ip[shell];
-    if (shell != d.m.vb) return E;
-    if (d.m.n_dos <= 0 || dE <= d.m.Ip[shell]) return E;
-    const int N = d.m.n_dos;
-    int M = (E < d.m.dosE[N - 1]) ? find_mono_1d(d.m.dosE, N, E) : N + 1;
-    if (M > 1) return E - dos_level(d, d.m.dosI[M - 2], p);
-    return E;
-}
-
-TRK_DEV double from_where_in_vb(const McDev &d, int has_E, double Elim, PState &p)
-{
-    if (d.m.n_dos <= 0) return 0.0;
-    const int N = d.m.n_dos;
-    if (!has_E) return dos_level(d, d.m.dosI[N - 1], p);
-    int M = (Elim < d.m.dosE[N - 1]) ? find_mono_1d(d.m.dosE, N, Elim) : N + 1;
-    if (M > 1) return dos_level(d, d.m.dosI[M - 2], p);
-    return 0.0;
-}
-
-// count_for_Auger_shells / Choose_for_Auger_shell, MC:1576-1647
-TRK_DEV double auger_count(const McDev &d, double NRG, int second)
-{
-    double c = 0.0;
-    for (int s = 0; s < d.m.nsh; ++s) {
-        double Ed = second ? 1.0e10 : (NRG - d.m.Ip[s]);
-        if ((NRG > d.m.Ip[s] + 1.0e-3) && (Ed >= d.m.Egap)) c = c + d.m.Nel[s];
-    }
-    return c;
-}
-
-TRK_DEV int auger_choose(const McDev &d, double NRG, double Shel, int second, int dflt)
-{
-    double cs = 0.0;
-    int found = dflt;
-    for (int s = 0; s < d.m.nsh; ++s) {
-        double Ed = second ? 1.0e10 : (NRG - d.m.Ip[s]);
-        if ((NRG > d.m.Ip[s] + 1.0e-3) && (Ed >= d.m.Egap)) {
-            cs = cs + d.m.Nel[s];
-            if (cs >= Shel) found = s;
-        }
-        if (cs >= Shel) break;
-    }
-    return found;
-}
-
-// ---------------------------------------------------------------- kinematics
-TRK_DEV double vel_el(double E) { return trk_sqrt(((2.0 * E) * kE) / kMe); }
-
-TRK_DEV double vel_hole(const PState &h)
-{
-    if (h.Mass < 1.0e6) {
-        if ((h.Ehkin < -1.0e-6) || (h.Mass < 1.0e-10)) return 0.0;
-        if (trk_fabs(h.Ehkin) < 1.0e-6) return 0.0;
-        return trk_sqrt(((2.0 * h.Ehkin) * kE) / (h.Mass * kMe));
-    }
-    return 0.0;
-}
-
-TRK_DEV double t_next(double t0, double MFP, double V) { return (V > 1.0e-10) ? t0 + (MFP / V) * 1.0e5 : 1.0e25; }
-TRK_DEV void cut_el(const McDev &d, PState &e) { if (e.E < d.m.cut_off) e.tn = 1.0e20; }
-TRK_DEV void cut_ho(const McDev &d, PState &h) { if ((h.Mass < 1.0e15) && (h.Ehkin < d.m.cut_off)) h.tn = 1.0e20; }
-
-// Hole_parameters + Assign_holes_mass + Get_time_of_next_event, MC:724-848;
-// draws come from the stream of the particle that owns the event (o).
-TRK_DEV void hole_params(const McDev &d, PState &h, double Eh, PState &o)
-{
-    if (h.sh == d.m.vb) {
-        double Etemp = h.Ehkin;
-        h.Ehkin = Eh - d.m.Egap;
-        h.E = d.m.Egap;
-        if (d.m.hole_mass > 0.0) h.Mass = d.m.hole_mass;
-        else h.Mass = d.m.dosM[find_mono_1d(d.m.dosE, d.m.n_dos, h.Ehkin) - 1];
-        if (h.Mass < 1.0e3) {
-            double HI = nfp_2d(d.EH, d.hole_imfp, d.nH, h.Ehkin);
-            double HE = (Etemp == (Eh - d.m.Egap)) ? 1.0e30 : nfp_2d(d.EHel, d.LHel, d.nHel, h.Ehkin);
-            double RN = rn(d, o);
-            double MFP = -trk_log(RN) / (1.0 / HI + 1.0 / HE);
-            h.tn = t_next(h.t0, MFP, vel_hole(h));
-            h.L = MFP;
-        } else {
-            h.L = 1.0e30;
-            h.tn = 1.0e30;
-        }
-    } else {
-        h.Mass = 1.0e29;
-        double RN = rn(d, o);
-        double nu = 1.0 / d.m.Auger[h.sh] + 1.0 / d.m.Radiat[h.sh];
-        h.tn = h.t0 - trk_log(RN) / nu;
-        h.L = 1.0e30;
-        h.E = Eh;
-        h.Ehkin = 0.0;
-    }
-}
-
-TRK_DEV double el_mfp(const McDev &d, double E, double RN)
-{
-    double lE = trk_log(E);
-    int N;
-    double IM = nfp_2d_ln(d.E, d.el_imfp, d.lnE, d.ln_el_imfp, d.nE, E, lE, &N);
-    double EM = nfp_2d_ln(d.Eel, d.Lel, d.lnEel, d.ln_Lel, d.nEel, E, lE, &N);
-    return -trk_log(RN) / (1.0 / IM + 1.0 / EM);
-}
-
-// el_mfp for the electron's own next flight: also keeps both mean free paths
-// and the elastic bracket for the electron's next event at this energy.
-TRK_DEV double el_mfp_keep(const McDev &d, double E, double RN, PState &e)
-{
-    double lE = trk_log(E);
-    int N;
-    double IM = nfp_2d_ln(d.E, d.el_imfp, d.lnE, d.ln_el_imfp, d.nE, E, lE, &N);
-    double EM = nfp_2d_ln(d.Eel, d.Lel, d.lnEel, d.ln_Lel, d.nEel, E, lE, &N);
-    e.cE = E; e.cIM = IM; e.cEM = EM; e.cN = N;
-    return -trk_log(RN) / (1.0 / IM + 1.0 / EM);
-}
-
-// ---------------------------------------------------------------- child allocation
-// d.overflow bits: list capacities, the per-particle event limit, NaN state.
-constexpr int kFlagPairs = 1, kFlagPhotons = 2, kFlagEvents = 4, kFlagNaN = 8;
-// Events one particle may process in one launch.  A history needs a few
-// thousand at most; the limit only guarantees that every wave ends.
-constexpr uint32_t kMaxEvents = 1u << 22;
-
-// Slot allocation aggregated over the active lanes of a wave: one atomic per
-// wave.  Every active lane must pass the same counter.
-TRK_DEV int wave_alloc(int *counter)
-{
-    unsigned long long mask = __ballot(1);
-    int lane = __lane_id();
-    int leader = __ffsll((long long)mask) - 1;
-    int rank = __popcll(mask & ((1ull << lane) - 1ull));
-    int base = 0;
-    if (lane == leader) base = atomicAdd(counter, __popcll(mask));
-    base = __shfl(base, leader);
-    return base + rank;
-}
-
-enum { K_EL = 0, K_HO = 1, K_PH = 2, K_ION = 3 };
-
-// Append one particle to the next round's list of its kind.
-TRK_DEV void emit(const McDev &d, int kind, const PState &q)
-{
-    int k = wave_alloc(d.out_cnt + kind);
-    if (k >= d.cap[kind]) { atomicOr(d.overflow, kind == K_PH ? kFlagPhotons : kFlagPairs); return; }
-    store(d.out[kind], k, q, kind == K_HO || kind == K_ION);
-}
-
-TRK_DEV void emit_pair(const McDev &d, const PState &e, const PState &h)
-{
-    emit(d, K_EL, e);
-    emit(d, K_HO, h);
-}
-
-TRK_DEV void emit_photon(const McDev &d, const PState &q) { emit(d, K_PH, q); }
-
-// ---------------------------------------------------------------- events
-// Electron_Monte_Carlo, MC:2253-2474
-TRK_DEV int electron_event(const McDev &d, PState &e, int rep)
-{
-    uint32_t ev = e.ne++;
-    double t_ev = e.tn;
-    double Eel = e.E;
-    double IMFP, EMFP;
-    int Nel;
-    if (Eel == e.cE) {
-        IMFP = e.cIM; EMFP = e.cEM; Nel = e.cN;
-    } else {
-        double lE = trk_log(Eel);
-        IMFP = nfp_2d_ln(d.E, d.el_imfp, d.lnE, d.ln_el_imfp, d.nE, Eel, lE, &Nel);
-        EMFP = nfp_2d_ln(d.Eel, d.Lel, d.lnEel, d.ln_Lel, d.nEel, Eel, lE, &Nel);
-    }
-    double RN = rn(d, e);
-    double L = e.L, th0 = e.th, ph0 = e.ph;
-    double sth0, cth0, sph0, cph0;
-    trk_sincos(th0, &sth0, &cth0);
-    trk_sincos(ph0, &sph0, &cph0);
-    double X = e.X + (L * sth0) * sph0;
-    double Y = e.Y + (L * sth0) * cph0;
-    double Z = e.Z + L * cth0;
-    double dE, theta, phi;
-    int kind;
-    if (RN * (1.0 / IMFP + 1.0 / EMFP) < 1.0 / IMFP) {
-        kind = EV_E_INEL;
-        int sh = which_shell(d, d.E, d.Lsh, d.nE, Eel, e);
-        double IMs = nfp_1d(d.E, d.Lsh + (size_t)sh * d.nE, d.nE, Eel);
-        dE = transfer_inel(d, Eel, sh, IMs, 0, e);
-        theta = trk_acos((Eel - dE) / trk_sqrt(Eel * (Eel - dE)));
-        if (theta != theta) theta = rn(d, e) * kPi;
-        phi = (2.0 * kPi) * rn(d, e);
-        double dE_cur = receives_E(d, dE, sh, e);
-        double MFP = el_mfp(d, dE_cur, rn(d, e));
-        double phi1, theta1;
-        new_angles(ph0, th0, cth0, kPi / 2.0 - theta, phi + kPi, phi1, theta1);
-        PState ne, nh;
-        init_child(ne, e.id, ev, 0, e.iter, 0);
-        init_child(nh, e.id, ev, 1, e.iter, 1);
-        ne.E = dE_cur; ne.t0 = e.tn; ne.X = X; ne.Y = Y; ne.Z = Z; ne.L = MFP; ne.th = theta1; ne.ph = phi1;
-        ne.tn = t_next(ne.t0, MFP, vel_el(ne.E));
-        cut_el(d, ne);
-        double hth = kPi * rn(d, e);
-        double hph = (2.0 * kPi) * rn(d, e);
-        nh.t0 = e.tn; nh.X = X; nh.Y = Y; nh.Z = Z; nh.sh = sh; nh.th = hth; nh.ph = hph;
-        hole_params(d, nh, dE - dE_cur, e);
-        cut_ho(d, nh);
-        emit_pair(d, ne, nh);
-    } else {
-        kind = EV_E_EL;
-        // nfp_1d brackets like nfp_2d unless the energy sits on a node or
-        // outside the grid; only then is the lookup repeated
-        double EMt;
-        if (Nel > 1 && d.Eel[Nel - 2] < Eel && Eel < d.Eel[Nel - 1]) EMt = EMFP;
-        else EMt = nfp_1d_ln(d.Eel, d.Lel, d.lnEel, d.ln_Lel, d.nEel, Eel, trk_log(Eel));
-        if (d.m.kind_of_EMFP == 1) {
-            dE = transfer_el(d, Eel, EMt, 0, e);
-        } else {
-            dE = mott_transfer_all(d, Eel, 1.0, e);
-        }
-        double mu = cos_theta_W(d, Eel, dE, 1.0 * kMe, e);
-        theta = trk_acos(mu);
-        phi = (2.0 * kPi) * rn(d, e);
-        tally_elat(d, rep, t_ev, X, Y, dE);
-    }
-    double MFP = el_mfp_keep(d, Eel - dE, rn(d, e), e);
-    double phi1, theta1;
-    new_angles(ph0, th0, cth0, theta, phi, phi1, theta1);
-    e.E = Eel - dE; e.t0 = e.tn; e.X = X; e.Y = Y; e.Z = Z; e.L = MFP; e.th = theta1; e.ph = phi1;
-    e.tn = t_next(e.t0, MFP, vel_el(e.E));
-    cut_el(d, e);
-    trace_ev(d, kind, e, ev, t_ev, e.E, 0.0, X, Y, Z);
-    return kind;
-}
-
-// check_hole_parameters, MC:682-721 (DOS walks clamped to the array)
-TRK_DEV double check_hole(const McDev &d, double Eel, double &dE, PState *el)
-{
-    double Ehole = Eel - dE;
-    int m = find_mono_1d(d.m.dosE, d.m.n_dos, Ehole);
-    if (d.m.dos[m - 1] < 1.0e-4) {
-        if (el) {
-            while (m > 1 && d.m.dos[m - 1] < 1.0e-4) m = m - 1;
-            double Eloc = d.m.dosE[m - 1];
-            el->E = el->E + (Ehole - Eloc);
-            Ehole = Eloc;
-        } else {
-            while (m < d.m.n_dos && d.m.dos[m - 1] < 1.0e-4) m = m + 1;
-            double Eloc = d.m.dosE[m - 1];
-            dE = dE + (Ehole - Eloc);
-            Ehole = Eloc;
-        }
-    }
-    return Ehole;
-}
-
-// Hole_Monte_Carlo, MC:2517-2869
-TRK_DEV int hole_event(const McDev &d, PState &h, int rep)
-{
-    uint32_t ev = h.ne++;
-    double t_ev = h.tn;
-    const double Egap = d.m.Egap;
-    int kind;
-    if (h.sh == d.m.vb) {
-        double Eel = h.Ehkin;
-        double HI, HE;
-        if (Eel == h.cE) {
-            HI = h.cIM; HE = h.cEM;
-        } else {
-            double lE = trk_log(Eel);
-            int N;
-            HI = nfp_2d_ln(d.EH, d.hole_imfp, d.lnEH, d.ln_hole_imfp, d.nH, Eel, lE, &N);
-            HE = nfp_2d_ln(d.EHel, d.LHel, d.lnEHel, d.ln_LHel, d.nHel, Eel, lE, &N);
-        }
-        double RN = rn(d, h);
-        double L = h.L, th0 = h.th, ph0 = h.ph;
-        double sth0, cth0, sph0, cph0;
-        trk_sincos(th0, &sth0, &cth0);
-        trk_sincos(ph0, &sph0, &cph0);
-        double X = h.X + (L * sth0) * sph0;
-        double Y = h.Y + (L * sth0) * cph0;
-        double Z = h.Z + L * cth0;
-        double dE, Ehole, hth1, hph1;
-        if ((RN * (1.0 / HI + 1.0 / HE) < 1.0 / HI) && (HI < 1.0e15)) {
-            kind = EV_H_INEL;
-            int sh = which_shell(d, d.EH, d.LHsh, d.nH, Eel, h);
-            double HIs = nfp_1d(d.EH, d.LHsh + (size_t)sh * d.nH, d.nH, Eel);
-            dE = transfer_inel(d, Eel, sh, HIs, 1, h);
-            double Mh = h.Mass * kMe;
-            double E11 = Eel - dE;
-            double hth = trk_acos(trk_sqrt(((((Mh + kMe) * (Mh + kMe)) / ((4.0 * Mh) * kMe)) * dE) / Eel));
-            double hph = (2.0 * kPi) * rn(d, h);
-            if (hth != hth) hth = kPi * rn(d, h);
-            hth1 = trk_acos(((Eel * (Mh - kMe)) + (E11 * (Mh + kMe))) / ((2.0 * Mh) * trk_sqrt(Eel * E11)));
-            hph1 = hph + kPi;
-            if (hth1 != hth1) hth1 = kPi * rn(d, h);
-            double dE_cur = receives_E(d, dE, sh, h);
-            double MFP = el_mfp(d, dE_cur, rn(d, h));
-            (void)rn(d, h);  // drawn and discarded, MC:2611
-            double phi1, theta1;
-            new_angles(ph0, th0, cth0, hth, hph, phi1, theta1);
-            PState ne, nh;
-            init_child(ne, h.id, ev, 0, h.iter, 0);
-            init_child(nh, h.id, ev, 1, h.iter, 1);
-            ne.E = dE_cur; ne.t0 = h.tn; ne.X = X; ne.Y = Y; ne.Z = Z; ne.L = MFP; ne.th = theta1; ne.ph = phi1;
-            ne.tn = t_next(ne.t0, MFP, vel_el(ne.E));
-            cut_el(d, ne);
-            double nth = kPi * rn(d, h);
-            double nph = (2.0 * kPi) * rn(d, h);
-            nh.t0 = h.tn; nh.X = X; nh.Y = Y; nh.Z = Z; nh.sh = sh; nh.th = nth; nh.ph = nph;
-            hole_params(d, nh, dE - dE_cur, h);
-            cut_ho(d, nh);
-            Ehole = check_hole(d, Eel, dE, &ne);
-            emit_pair(d, ne, nh);
-        } else {
-            kind = EV_H_EL;
-            double HEt = nfp_1d(d.EHel, d.LHel, d.nHel, Eel);
-            if (d.m.kind_of_EMFP == 1) {
-                dE = transfer_el(d, Eel, HEt, 1, h);
-            } else {
-                dE = mott_transfer_all(d, Eel, h.Mass, h);
-            }
-            double mu = cos_theta_W(d, Eel, dE, h.Mass * kMe, h);
-            hth1 = trk_acos(mu);
-            hph1 = (2.0 * kPi) * rn(d, h);
-            Ehole = check_hole(d, Eel, dE, nullptr);
-            tally_elat(d, rep, t_ev, X, Y, dE);
-        }
-        double hph2, hth2;
-        new_angles(ph0, th0, cth0, hth1, hph1, hph2, hth2);
-        h.t0 = h.tn; h.X = X; h.Y = Y; h.Z = Z; h.th = hth2; h.ph = hph2;
-        hole_params(d, h, Ehole + Egap, h);
-        cut_ho(d, h);
-        trace_ev(d, kind, h, ev, t_ev, h.Ehkin, h.E, X, Y, Z);
-        return kind;
-    }
-    // deep shell, MC:2739-2868
-    double RN = rn(d, h);
-    double tA = d.m.Auger[h.sh], tR = d.m.Radiat[h.sh];
-    if (RN * (1.0 / tA + 1.0 / tR) < 1.0 / tA) {
-        kind = EV_AUGER;
-        int SHL = h.sh;
-        int Sh2 = -1;
-        double Ee = -1.0e-10, E_new2 = 0.0;
-        double coun = auger_count(d, d.m.Ip[SHL], 0);
-        double Shel = rn(d, h) * coun;
-        int Sh1 = auger_choose(d, d.m.Ip[SHL], Shel, 0, SHL);
-        double dEc = 0.0;
-        if (d.m.n_dos > 0 && Sh1 == d.m.vb) dEc = from_where_in_vb(d, 0, 0.0, h);
-        double E_new1 = dEc + d.m.Ip[Sh1];
-        double Ediff = d.m.Ip[SHL] - E_new1;
-        coun = auger_count(d, Ediff, 1);
-        if (coun > 0.0) {
-            Shel = rn(d, h) * coun;
-            Sh2 = auger_choose(d, Ediff, Shel, 1, -1);
-            dEc = 0.0;
-            if (Sh2 >= 0 && d.m.n_dos > 0 && Sh2 == d.m.vb) dEc = from_where_in_vb(d, 1, Ediff - d.m.Ip[Sh2], h);
-            E_new2 = dEc + ((Sh2 >= 0) ? d.m.Ip[Sh2] : 0.0);
-            Ee = Ediff - E_new2;
-        }
-        if (Sh2 >= 0) {
-            double hth = kPi * rn(d, h);
-            double hph = (2.0 * kPi) * rn(d, h);
-            h.t0 = h.tn; h.sh = Sh1; h.th = hth; h.ph = hph;
-            hole_params(d, h, E_new1, h);
-            cut_ho(d, h);
-            PState ne, nh;
-            init_child(nh, h.id, ev, 0, h.iter, 1);
-            init_child(ne, h.id, ev, 1, h.iter, 0);
-            double nth = kPi * rn(d, h);
-            double nph = (2.0 * kPi) * rn(d, h);
-            nh.t0 = h.t0; nh.X = h.X; nh.Y = h.Y; nh.Z = h.Z; nh.sh = Sh2; nh.th = nth; nh.ph = nph;
-            hole_params(d, nh, E_new2, h);
-            cut_ho(d, nh);
-            double MFP = el_mfp(d, Ee, rn(d, h));
-            double phi1 = (2.0 * kPi) * rn(d, h);
-            double theta1 = kPi * rn(d, h);
-            ne.E = Ee; ne.t0 = h.t0; ne.X = h.X; ne.Y = h.Y; ne.Z = h.Z; ne.L = MFP; ne.th = theta1; ne.ph = phi1;
-            ne.tn = t_next(ne.t0, MFP, vel_el(ne.E));
-            cut_el(d, ne);
-            emit_pair(d, ne, nh);
-        } else {
-            h.t0 = t_ev;
-            h.tn = 1.0e21;
-        }
-    } else {
-        kind = EV_RAD;
-        int SHL = h.sh;
-        double coun = auger_count(d, d.m.Ip[SHL], 1);
-        double Shel = rn(d, h) * coun;
-        int Sh1 = auger_choose(d, d.m.Ip[SHL], Shel, 1, SHL);
-        double dEc = 0.0;
-        if (d.m.n_dos > 0 && Sh1 == d.m.vb) dEc = from_where_in_vb(d, 0, 0.0, h);
-        double E_new1 = dEc + d.m.Ip[Sh1];
-        double dE = d.m.Ip[SHL] - E_new1;
-        double hth = kPi * rn(d, h);
-        double hph = (2.0 * kPi) * rn(d, h);
-        h.t0 = h.tn; h.sh = Sh1; h.th = hth; h.ph = hph;
-        hole_params(d, h, E_new1, h);
-        cut_ho(d, h);
-        PState q;
-        init_child(q, h.id, ev, 0, h.iter, 0);
-        double IM = nfp_2d(d.EP, d.phot_imfp, d.nP, dE);
-        double MFP = -trk_log(rn(d, h)) * IM;
-        double phi1 = (2.0 * kPi) * rn(d, h);
-        double theta1 = kPi * rn(d, h);
-        q.E = dE; q.t0 = h.t0; q.X = h.X; q.Y = h.Y; q.Z = h.Z; q.L = MFP; q.th = theta1; q.ph = phi1;
-        q.tn = t_next(q.t0, MFP, kC);
-        emit_photon(d, q);
-    }
-    trace_ev(d, kind, h, ev, t_ev, h.Ehkin, h.E, h.X, h.Y, h.Z);
-    return kind;
-}
-
-// Photon_Monte_Carlo, MC:2873-2965
-TRK_DEV void photon_event(const McDev &d, PState &q)
-{
-    uint32_t ev = q.ne++;
-    double t_ev = q.tn;
-    double Eel = q.E;
-    double L = q.L, th0 = q.th, ph0 = q.ph;
-    double sth0, cth0, sph0, cph0;
-    trk_sincos(th0, &sth0, &cth0);
-    trk_sincos(ph0, &sph0, &cph0);
-    double X = q.X + (L * sth0) * sph0;
-    double Y = q.Y + (L * sth0) * cph0;
-    double Z = q.Z + L * cth0;
-    int sh = which_shell(d, d.EP, d.LPsh, d.nP, Eel, q);
-    double dE_cur = receives_E(d, Eel, sh, q);
-    double MFP = el_mfp(d, dE_cur, rn(d, q));
-    double phi1, theta1;
-    new_angles(q.ph, q.th, cth0, kPi / 2.0, 0.0, phi1, theta1);
-    PState ne, nh;
-    init_child(ne, q.id, ev, 0, q.iter, 0);
-    init_child(nh, q.id, ev, 1, q.iter, 1);
-    ne.E = dE_cur; ne.t0 = q.tn; ne.X = X; ne.Y = Y; ne.Z = Z; ne.L = MFP; ne.th = theta1; ne.ph = phi1;
-    ne.tn = t_next(ne.t0, MFP, vel_el(ne.E));
-    cut_el(d, ne);
-    double hth = kPi * rn(d, q);
-    double hph = (2.0 * kPi) * rn(d, q);
-    nh.t0 = q.tn; nh.X = X; nh.Y = Y; nh.Z = Z; nh.sh = sh; nh.th = hth; nh.ph = hph;
-    hole_params(d, nh, Eel - dE_cur, q);
-    cut_ho(d, nh);
-    emit_pair(d, ne, nh);
-    q.E = 0.0; q.t0 = 1.0e27; q.tn = 1.0e27; q.X = 0.0; q.Y = 0.0; q.Z = 0.0; q.L = 1.0e26; q.th = 0.0; q.ph = 0.0;
-    trace_ev(d, EV_PHOT, q, ev, t_ev, Eel, 0.0, X, Y, Z);
-}
-
-// ---------------------------------------------------------------- SHI
-// SHI_energy_transfer, MC:1719-1780 (CDF branch)
-TRK_DEV double shi_transfer(const McDev &d, int shell, PState &I)
-{
-    const int N = d.sd_n[shell];
-    const double *E = d.sd_E + d.sd_off[shell], *L = d.sd_L + d.sd_off[shell];
-    double RN = rn(d, I);
-    if (N < 1) return TRK_NAN;
-    double Ecur = d.m.Ip[shell];
-    int M = find_mono_1d(E, N, d.m.Ip[shell]);
-    double dL;
-    if (M > 1) dL = (L[M - 2] > 1.0e-10) ? interpolate(5, E[M - 2], E[M - 1], L[M - 2], L[M - 1], Ecur)
-                                          : interpolate(1, E[M - 2], E[M - 1], L[M - 2], L[M - 1], Ecur);
-    else dL = L[0];
-    double Tot = ((dL > 0.0) && (L[N - 1] > 0.0)) ? 1.0 / dL + RN * (1.0 / L[N - 1] - 1.0 / dL) : 1.5e21;
-    int Nt;
-    if (Tot < 1.0e20) {
-        // SHI_energy_transfer scans linearly for the first i with
-        // 1/L(i) >= Tot among i < N (Find_in_1D_array, RF:3348-3358); on a
-        // row whose inverse never decreases a lower-bound bisection finds the
-        // same index in log2(N) steps (the rows have ~10^3-10^4 nodes).
-        const double *inv = d.sd_inv + d.sd_off[shell];
-        if (!d.sd_nonmono[shell]) {
-            int lo = 0, hi = N - 1;
-            while (lo < hi) {
-                int mid = (lo + hi) >> 1;
-                if (inv[mid] < Tot) lo = mid + 1;
-                else hi = mid;
-            }
-            Nt = lo + 1;
-        } else {
-            int i = 1;
-            while (i < N && inv[i - 1] < Tot) i++;
-            Nt = i;
-        }
-    } else {
-        Nt = M;
-    }
-    if (Nt > M) return interpolate(5, 1.0 / L[Nt - 2], 1.0 / L[Nt - 1], E[Nt - 2], E[Nt - 1], Tot);
-    return d.m.Ip[shell];
-}
-
-TRK_DEV double shi_zeff_dev(const McDev &d, double E)
-{
-    double vp = (E > 0.0) ? trk_sqrt(((2.0 * E) * kE) / (d.m.ion_mass * kMp)) : 0.0;
-    double Zp = (double)d.m.ion_Z;
-    double Zt = d.m.Zt;
-    double v0 = trk_sqrt(((2.0 * kRy) * kE) / kMe);
-    switch (d.m.ion_kind_zeff) {
-    case 1:
-        return Zp * (1.0 - trk_exp(-((vp / v0) / trk_pow(Zp, 0.66666666))));
-    case 2:
-        return Zp * trk_pow(1.0 + trk_pow(vp / (((trk_pow(Zp, 0.45) * v0) * 4.0) / 3.0), -1.0 / 0.6), -0.6);
-    case 3: {
-        double c1 = 1.0 - 0.26 * trk_exp(-(Zt / 11.0) - ((Zt - Zp) * (Zt - Zp)) / 9.0);
-        double vpvo = (trk_pow(Zp, -0.543) * vp) / v0;
-        double c2 = 1.0 + (0.03 * vpvo) * trk_log(Zt);
-        double x = c1 * trk_pow((vpvo / c2) / 1.54, 1.0 + 1.83 / Zp);
-        double x2 = x * x, x4 = x2 * x2;
-        return (Zp * ((8.29 * x) + x4)) / ((((0.06 / x) + 4.0) + (7.4 * x)) + x4);
-    }
-    case 4:
-        return d.m.ion_fixed_zeff;
-    default:
-        return Zp * (1.0 - trk_exp(-(((vp * 125.0) / kC) / trk_pow(Zp, 0.66666666))));
-    }
-}
-
-// The ion of every iteration starts at the surface with its first free
-// flight drawn (MC:564); one thread per iteration writes it to the ion list.
-__global__ void __launch_bounds__(64) k_ion_init(McDev d, int nb, uint32_t first_iter)
-{
-    int b = blockIdx.x * blockDim.x + threadIdx.x;
-    if (b >= nb) return;
-    PState I;
-    I.E = d.m.ion_E; I.t0 = 0.0; I.X = 0.0; I.Y = 0.0; I.Z = 0.0; I.th = 0.0; I.ph = 0.0;
-    I.Ehkin = 0.0; I.Mass = d.m.ion_Zeff0; I.sh = -1;   // Mass holds Zeff for the ion
-    I.id = 0; I.nd = 0; I.ne = 0; I.iter = first_iter + (uint32_t)b;
-    I.cb = 0u;
-    const double MSHI = d.m.ion_mass * kMp;
-    double SL = nfp_2d(d.ES, d.shi_path, d.nS, I.E);
-    SL = -SL * trk_log(rn(d, I));
-    I.tn = t_next(0.0, SL, trk_sqrt(((2.0 * I.E) * kE) / MSHI));
-    I.L = SL;
-    store(d.in[K_ION], b, I, 1);
-}
-
-// One ionising collision of the ion (SHI_Monte_Carlo, MC:2153-2249): shell,
-// energy transfer, next free flight, then the delta electron and the hole.
-TRK_DEV void shi_event(const McDev &d, PState &I)
-{
-    const double MSHI = d.m.ion_mass * kMp;
-    uint32_t ev = I.ne++;
-    double t_ev = I.tn;
-    int sh = which_shell(d, d.ES, d.LSsh, d.nS, I.E, I);
-    double dE = shi_transfer(d, sh, I);
-    double SI = nfp_2d(d.ES, d.shi_path, d.nS, I.E);
-    SI = -SI * trk_log(rn(d, I));
-    double Z = I.Z + I.L;
-    I.E = I.E - dE;
-    I.t0 = I.tn;
-    I.Z = Z;
-    I.L = SI;
-    I.tn = t_next(I.t0, SI, trk_sqrt(((2.0 * I.E) * kE) / MSHI));
-    double Zeff = shi_zeff_dev(d, I.E);
-    I.Mass = Zeff;
-    PState ne, nh;
-    init_child(ne, I.id, ev, 0, I.iter, 0);
-    init_child(nh, I.id, ev, 1, I.iter, 1);
-    double dE_cur = receives_E(d, dE, sh, I);
-    double theta = (I.E <= 0.0) ? kPi / 2.0
-                                : trk_acos(trk_sqrt(((((MSHI + kMe) * (MSHI + kMe)) / ((4.0 * MSHI) * kMe)) * dE) / I.E));
-    double phi = (2.0 * kPi) * rn(d, I);
-    double MFP = el_mfp(d, dE_cur, rn(d, I));
-    double A = 1.0 + MSHI / kMe;
-    double Lb = (((kA0 * Zeff) * kRy) / I.E) * trk_sqrt(((((4.0 * I.E) / dE) * MSHI) / kMe) - A * A);
-    double sphi, cphi;
-    trk_sincos(phi, &sphi, &cphi);
-    double X = I.X + Lb * sphi;
-    double Y = I.Y + Lb * cphi;
-    ne.E = dE_cur; ne.t0 = I.t0; ne.X = X; ne.Y = Y; ne.Z = Z; ne.L = MFP; ne.th = theta; ne.ph = phi;
-    ne.tn = t_next(ne.t0, MFP, vel_el(ne.E));
-    cut_el(d, ne);
-    double hth = kPi * rn(d, I);
-    double hph = (2.0 * kPi) * rn(d, I);
-    nh.t0 = I.t0; nh.X = X; nh.Y = Y; nh.Z = Z; nh.sh = sh; nh.th = hth; nh.ph = hph;
-    hole_params(d, nh, dE - dE_cur, I);
-    cut_ho(d, nh);
-    if (Z >= d.m.Layer)
-        I.tn = 1.0e16;
-    emit_pair(d, ne, nh);
-    trace_ev(d, EV_SHI, I, ev, t_ev, I.E, Zeff, I.X, I.Y, I.Z);
-}
-
-// Category of a live particle's next event, read from the draw that event
-// will use first (the draw is only looked at here; the event consumes it):
-// the electron el/inel test (MC:2301-2309), the VB-hole test (MC:2568-2586),
-// deep-hole decay, photoabsorption and ion collisions.  Fills the free-path
-// cache it needs.
-enum { C_E_EL = 0, C_E_INEL, C_H_EL, C_H_INEL, C_DEEP, C_PHOT, C_ION, C_COUNT };
-
-TRK_DEV int peek_category(const McDev &d, PState &p, int kind)
-{
-    if (kind == K_PH) return C_PHOT;
-    if (kind == K_ION) return C_ION;
-    if (kind == K_HO && p.sh != d.m.vb) return C_DEEP;
-    const double RN = rn_at(d, p, p.nd);
-    if (kind == K_EL) {
-        if (!(p.E == p.cE)) {
-            double lE = trk_log(p.E);
-            int N;
-            p.cIM = nfp_2d_ln(d.E, d.el_imfp, d.lnE, d.ln_el_imfp, d.nE, p.E, lE, &N);
-            p.cEM = nfp_2d_ln(d.Eel, d.Lel, d.lnEel, d.ln_Lel, d.nEel, p.E, lE, &N);
-            p.cN = N;
-            p.cE = p.E;
-        }
-        return (RN * (1.0 / p.cIM + 1.0 / p.cEM) < 1.0 / p.cIM) ? C_E_INEL : C_E_EL;
-    }
-    if (!(p.Ehkin == p.cE)) {
-        double lE = trk_log(p.Ehkin);
-        int N;
-        p.cIM = nfp_2d_ln(d.EH, d.hole_imfp, d.lnEH, d.ln_hole_imfp, d.nH, p.Ehkin, lE, &N);
-        p.cEM = nfp_2d_ln(d.EHel, d.LHel, d.lnEHel, d.ln_LHel, d.nHel, p.Ehkin, lE, &N);
-        p.cN = N;
-        p.cE = p.Ehkin;
-    }
-    return ((RN * (1.0 / p.cIM + 1.0 / p.cEM) < 1.0 / p.cIM) && (p.cIM < 1.0e15)) ? C_H_INEL : C_H_EL;
-}
-
-// A wave runs a category of events only when at least this many of its lanes
-// want it (or when it is the most wanted one); the other lanes wait with
-// their state untouched.  Rare, long branches (ionisation, decays) then run
-// for many lanes at once instead of stalling the wave at almost every step.
-constexpr int kBatchLanes = 16;
-constexpr int kMaxWait = 8;
-
-// Events one particle may take in one round.  A particle with more left is
-// written to the next round's list; this bounds the tail of every launch by
-// the cost of kRoundEvents events instead of the longest history.
-constexpr uint32_t kRoundEvents = 512;
-
-// One round of transport.  Items [0, n_e) are electrons, then holes, photons
-// and ions.  A lane takes the next item from a shared cursor whenever its
-// particle is finished (below the cut-off, absorbed, past the last time
-// node) or has used its event budget for the round, so waves stay full until
-// the round's list is used up; every lane leaves once the cursor has passed
-// the last item.  Children and unfinished particles go to the next round's
-// lists.  A particle's snapshot tallies for the interval up to its next
-// event are added once, when that interval is known (when the particle is
-// loaded and after each event it takes in this round).
-// The small tables every event searches (energy grids, fused free paths and
-// their logs, per-shell free paths, radial and time grids, the VB DOS) are
-// copied into LDS once per workgroup: a bisection step then costs an LDS
-// access instead of an L2 round trip.  Order of the staged arrays:
-//  0-3 E, El_IMFP, ln E, ln El_IMFP      4-7 elastic E, L, ln E, ln L
-//  8-11 hole E, Hole_IMFP, logs          12-15 hole elastic E, L, logs
-//  16 Out_R  17 time nodes  18-21 DOS E, DOS, int_DOS, Eff_m
-//  22 per-shell electron L  23 per-shell hole L
-extern __shared__ double lds_tab[];
-
-TRK_DEV void stage_tables(const McDev &d, McDev &l)
-{
-    const double *src[24] = {d.E, d.el_imfp, d.lnE, d.ln_el_imfp, d.Eel, d.Lel, d.lnEel, d.ln_Lel,
-                             d.EH, d.hole_imfp, d.lnEH, d.ln_hole_imfp, d.EHel, d.LHel, d.lnEHel, d.ln_LHel,
-                             d.R, d.tim_glob, d.m.dosE, d.m.dos, d.m.dosI, d.m.dosM, d.Lsh, d.LHsh};
-#pragma unroll
-    for (int a = 0; a < 24; ++a) {
-        const int n = d.lds_n[a];
-        double *dst = lds_tab + d.lds_off[a];
-        for (int i = threadIdx.x; i < n; i += blockDim.x) dst[i] = src[a][i];
-    }
-    __syncthreads();
-#define TRK_STAGED(a, field) if (d.lds_n[a]) l.field = lds_tab + d.lds_off[a]
-    TRK_STAGED(0, E); TRK_STAGED(1, el_imfp); TRK_STAGED(2, lnE); TRK_STAGED(3, ln_el_imfp);
-    TRK_STAGED(4, Eel); TRK_STAGED(5, Lel); TRK_STAGED(6, lnEel); TRK_STAGED(7, ln_Lel);
-    TRK_STAGED(8, EH); TRK_STAGED(9, hole_imfp); TRK_STAGED(10, lnEH); TRK_STAGED(11, ln_hole_imfp);
-    TRK_STAGED(12, EHel); TRK_STAGED(13, LHel); TRK_STAGED(14, lnEHel); TRK_STAGED(15, ln_LHel);
-    TRK_STAGED(16, R); TRK_STAGED(17, tim_glob); TRK_STAGED(18, m.dosE); TRK_STAGED(19, m.dos);
-    TRK_STAGED(20, m.dosI); TRK_STAGED(21, m.dosM); TRK_STAGED(22, Lsh); TRK_STAGED(23, LHsh);
-#undef TRK_STAGED
-}
-
-constexpr int kTransportBlock = 768;   // 12 waves: one workgroup per CU at 3 waves per SIMD
-
-__global__ void __launch_bounds__(kTransportBlock) k_transport(McDev din)
-{
-    McDev d = din;
-    stage_tables(din, d);
-    // ions first: they start every round at once and feed all other work
-    const int n0 = d.n_in[K_ION], n1 = n0 + d.n_in[K_EL], n2 = n1 + d.n_in[K_HO], total = n2 + d.n_in[K_PH];
-    unsigned long long c_ei = 0, c_ee = 0, c_hi = 0, c_he = 0, c_au = 0, c_ra = 0, c_ph = 0, c_shi = 0;
-    PState p;
-    int kind = 0, rep = 0, cat = 0;
-    uint32_t budget = 0;
-    int waited = 0;
-    bool have = false;
-    for (;;) {
-        if (!have) {
-            int t = wave_alloc(d.cursor);
-            if (t >= total) break;
-            if (t < n0) { kind = K_ION; load(d.in[K_ION], t, p, 1); }
-            else if (t < n1) { kind = K_EL; load(d.in[K_EL], t - n0, p, 0); }
-            else if (t < n2) { kind = K_HO; load(d.in[K_HO], t - n1, p, 1); }
-            else { kind = K_PH; load(d.in[K_PH], t - n2, p, 0); }
-            rep = (int)(p.iter % (uint32_t)d.T.R);
-            budget = kRoundEvents;
-            waited = 0;
-            if (kind != K_ION) segment(d, p, kind, rep);
-            if (p.tn < d.T_last) {
-                if (p.E != p.E) atomicOr(d.overflow, kFlagNaN);
-                else { have = true; cat = peek_category(d, p, kind); }
-            }
-        }
-        // choose the categories this wave runs now: every category wanted by
-        // kBatchLanes lanes, the most wanted one, the ions (few, and the root
-        // of all other work) and any category a lane has waited kMaxWait
-        // steps for
-        int best = -1, bestn = 0;
-        unsigned run = 1u << C_ION;
-#pragma unroll
-        for (int c = 0; c < C_COUNT; ++c) {
-            int n = __popcll(__ballot(have && cat == c));
-            if (n >= kBatchLanes) run |= 1u << c;
-            if (n > bestn) { bestn = n; best = c; }
-            if (__ballot(have && cat == c && waited >= kMaxWait)) run |= 1u << c;
-        }
-        if (best >= 0) run |= 1u << best;
-        if (!have) continue;
-        if (!((run >> cat) & 1u)) { ++waited; continue; }
-        waited = 0;
-
-        if (kind == K_EL) {
-            int k = electron_event(d, p, rep);
-            if (k == EV_E_INEL) c_ei++; else c_ee++;
-        } else if (kind == K_HO) {
-            int k = hole_event(d, p, rep);
-            if (k == EV_H_INEL) c_hi++;
-            else if (k == EV_H_EL) c_he++;
-            else if (k == EV_AUGER) c_au++;
-            else c_ra++;
-        } else if (kind == K_PH) {
-            photon_event(d, p);
-            c_ph++;
-            have = false;
-            continue;
-        } else {
-            shi_event(d, p);
-            c_shi++;
-        }
-        if (p.ne >= kMaxEvents) { atomicOr(d.overflow, kFlagEvents); have = false; continue; }
-        if (p.E != p.E || p.tn != p.tn) { atomicOr(d.overflow, kFlagNaN); have = false; continue; }
-        if (--budget == 0 && p.tn < d.T_last) {
-            // continues next round; its next interval is tallied then.  The
-            // append is made per kind so that every lane in one wave-wide
-            // allocation uses the same list counter.
-#pragma unroll
-            for (int k = 0; k < 4; ++k)
-                if (kind == k) emit(d, k, p);
-            have = false;
-            continue;
-        }
-        if (kind != K_ION) segment(d, p, kind, rep);
-        if (!(p.tn < d.T_last)) { have = false; continue; }
-        cat = peek_category(d, p, kind);
-    }
-    if (c_ei) atomicAdd(&d.counters[EV_E_INEL], c_ei);
-    if (c_ee) atomicAdd(&d.counters[EV_E_EL], c_ee);
-    if (c_hi) atomicAdd(&d.counters[EV_H_INEL], c_hi);
-    if (c_he) atomicAdd(&d.counters[EV_H_EL], c_he);
-    if (c_au) atomicAdd(&d.counters[EV_AUGER], c_au);
-    if (c_ra) atomicAdd(&d.counters[EV_RAD], c_ra);
-    if (c_ph) atomicAdd(&d.counters[EV_PHOT], c_ph);
-    if (c_shi) atomicAdd(&d.counters[EV_SHI], c_shi);
-}
+namespace mx {
+#define TRK_MX 1
+#include "trk_events.inc"
+#undef TRK_MX
+}  // namespace mx
 
 // ---------------------------------------------------------------- per-iteration tallies
 // Prefix sums over time of [B][Nt][ncol] difference arrays, one thread per column.
@@ -1730,6 +456,7 @@ int run_mc(trk_ctx *c, int64_t nmc, uint64_t seed, int64_t first_iter, const dou
            int NR, trk_tallies *out)
 {
     auto tstart = std::chrono::steady_clock::now();
+    const bool mixed = c->hm.o.precision == 1;   // FP32 elementary functions in the event code
     const HostMat &h = c->hm;
     if (!c->mc) c->mc = new McState();
     McState *s = c->mc;
@@ -1878,11 +605,14 @@ int run_mc(trk_ctx *c, int64_t nmc, uint64_t seed, int64_t first_iter, const dou
             off += n;
         }
         s->lds_bytes = off * sizeof(double);
+        const void *kt = mixed ? (const void *)mx::k_transport : (const void *)fp64::k_transport;
         if (s->lds_bytes > 65536)
-            HIPC(hipFuncSetAttribute((const void *)k_transport, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                     (int)s->lds_bytes));
+            HIPC(hipFuncSetAttribute(kt, hipFuncAttributeMaxDynamicSharedMemorySize, (int)s->lds_bytes));
         int per_cu = 0;
-        HIPC(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_transport, kTransportBlock, s->lds_bytes));
+        if (mixed)
+            HIPC(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, mx::k_transport, kTransportBlock, s->lds_bytes));
+        else
+            HIPC(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fp64::k_transport, kTransportBlock, s->lds_bytes));
         if (per_cu < 1) per_cu = 1;
         s->max_blocks = (int64_t)per_cu * s->ncu;
     }
@@ -1910,7 +640,8 @@ int run_mc(trk_ctx *c, int64_t nmc, uint64_t seed, int64_t first_iter, const dou
         for (int k = 0; k < 4; ++k) d.in[k] = s->L[cur][k];
         HIPC(hipEventRecord(s->ev0, st));
         if (c->prof) HIPC(hipEventRecord(c->pe[0], st));
-        k_ion_init<<<(nb + 63) / 64, 64, 0, st>>>(d, nb, (uint32_t)(first_iter + b0));
+        if (mixed) mx::k_ion_init<<<(nb + 63) / 64, 64, 0, st>>>(d, nb, (uint32_t)(first_iter + b0));
+        else fp64::k_ion_init<<<(nb + 63) / 64, 64, 0, st>>>(d, nb, (uint32_t)(first_iter + b0));
         HIPC(hipGetLastError());
         if (c->prof) HIPC(hipEventRecord(c->pe[1], st));
         launches++;
@@ -1954,7 +685,8 @@ int run_mc(trk_ctx *c, int64_t nmc, uint64_t seed, int64_t first_iter, const dou
             d.cursor = s->d_counts + 8;
             HIPC(hipMemsetAsync(d.cursor, 0, sizeof(int), st));
             if (c->prof) HIPC(hipEventRecord(c->pe[0], st));
-            k_transport<<<(unsigned)nblk, kTransportBlock, s->lds_bytes, st>>>(d);
+            if (mixed) mx::k_transport<<<(unsigned)nblk, kTransportBlock, s->lds_bytes, st>>>(d);
+            else fp64::k_transport<<<(unsigned)nblk, kTransportBlock, s->lds_bytes, st>>>(d);
             HIPC(hipGetLastError());
             if (c->prof) { HIPC(hipEventRecord(c->pe[1], st)); pending = 1; }
             launches++;
