@@ -90,7 +90,7 @@ def tally_errors(ref, mix, min_frac=1.0e-2):
             worst = max(worst, e["max_rel"])
         out[k] = e
     te_a, te_b = float(ref["Out_tot_E"].sum()), float(mix["Out_tot_E"].sum())
-    return worst, out, abs(te_b - te_a) / abs(te_a), photons
+    return worst, out, (abs(te_b - te_a) / abs(te_a)) if te_a else 0.0, photons
 
 
 def run_leg(tg, abi, args, precision, first_iter):

@@ -83,6 +83,7 @@ struct McDev {
     double T_last, Tim, cutoff_eff;
     int Nt; const double *tim_glob;
     double inv_dt;                      // 1/dt of a linear time grid, 0 otherwise
+    int hints;                          // 1: the electron and hole energy grids and the time grid are strictly increasing
     // tables copied into each transport workgroup's LDS (count and offset in
     // doubles; count 0 = read from global memory), see kStage* below
     int lds_n[24], lds_off[24];
@@ -522,6 +523,25 @@ int run_mc(trk_ctx *c, int64_t nmc, uint64_t seed, int64_t first_iter, const dou
     d.cutoff_eff = (h.m.cut_off > 0.0) ? h.m.cut_off : 0.0;
     d.Nt = Nt; d.tim_glob = s->d_tim;
     d.inv_dt = (h.o.dt_flag <= 0 && h.o.dt > 0.0) ? 1.0 / h.o.dt : 0.0;
+    {
+        // a particle's last bracket is tried before a search only on strictly
+        // increasing grids, where it gives the search's own result
+        d.hints = 1;
+        const int kinds[4] = {TRK_TAB_E_INEL, TRK_TAB_E_EL, TRK_TAB_H_INEL, TRK_TAB_H_EL};
+        for (int k : kinds) {
+            const TableDev &t = c->ts.tab[k];
+            if (t.n <= 0 || !t.E) continue;
+            std::vector<double> g((size_t)t.n);
+            HIPC(hipMemcpyAsync(g.data(), t.E, sizeof(double) * t.n, hipMemcpyDeviceToHost, st));
+            HIPC(hipStreamSynchronize(st));
+            for (int i = 1; i < t.n; ++i)
+                if (!(g[i - 1] < g[i])) d.hints = 0;
+        }
+        // and the time grid (a flight with no node in (t0, tn] is recognised
+        // from the first node after t0)
+        for (int i = 1; i < Nt; ++i)
+            if (!(h.tim_glob[i - 1] < h.tim_glob[i])) d.hints = 0;
+    }
     {
         const TableSet &ts = c->ts;
         d.lnE = ts.ln_E[TRK_TAB_E_INEL]; d.ln_el_imfp = ts.ln_el_imfp;
