@@ -66,6 +66,9 @@ struct Spec {
     unsigned long long *dcs;       // [B][Nt][2]    sum of v*L/3 of mobile holes, fixed point
 };
 
+// tables staged in each transport workgroup's LDS (see stage_tables)
+constexpr int kStaged = 26;
+
 struct McDev {
     MatDev m;
     // tables
@@ -78,6 +81,9 @@ struct McDev {
     const int *sd_n; const int64_t *sd_off; const double *sd_E, *sd_L, *sd_inv; const int *sd_nonmono;
     // logs of grids and fused tables (interpolation without per-node logs)
     const double *lnE, *ln_el_imfp, *lnEel, *ln_Lel, *lnEH, *ln_hole_imfp, *lnEHel, *ln_LHel;
+    // logs of the per-shell paths and of the cumulative transfer rows
+    const double *ln_Lsh, *ln_LHsh;
+    const double *ln_rhw, *ln_rlam, *ln_rhw_h, *ln_rlam_h, *ln_rhw_el, *ln_rlam_el, *ln_rhw_hel, *ln_rlam_hel;
     // run
     uint64_t seed;
     double T_last, Tim, cutoff_eff;
@@ -86,7 +92,7 @@ struct McDev {
     int hints;                          // 1: the electron and hole energy grids and the time grid are strictly increasing
     // tables copied into each transport workgroup's LDS (count and offset in
     // doubles; count 0 = read from global memory), see kStage* below
-    int lds_n[24], lds_off[24];
+    int lds_n[kStaged], lds_off[kStaged];
     int NR; const double *R;
     int include_photons;
     // lists, by kind: electrons, holes, photons, ions
@@ -548,6 +554,11 @@ int run_mc(trk_ctx *c, int64_t nmc, uint64_t seed, int64_t first_iter, const dou
         d.lnEel = ts.ln_E[TRK_TAB_E_EL]; d.ln_Lel = ts.ln_Lel;
         d.lnEH = ts.ln_E[TRK_TAB_H_INEL]; d.ln_hole_imfp = ts.ln_hole_imfp;
         d.lnEHel = ts.ln_E[TRK_TAB_H_EL]; d.ln_LHel = ts.ln_LHel;
+        d.ln_Lsh = ts.ln_L[TRK_TAB_E_INEL]; d.ln_LHsh = ts.ln_L[TRK_TAB_H_INEL];
+        d.ln_rhw = ts.ln_hw[TRK_TAB_E_INEL]; d.ln_rlam = ts.ln_lam[TRK_TAB_E_INEL];
+        d.ln_rhw_h = ts.ln_hw[TRK_TAB_H_INEL]; d.ln_rlam_h = ts.ln_lam[TRK_TAB_H_INEL];
+        d.ln_rhw_el = ts.ln_hw[TRK_TAB_E_EL]; d.ln_rlam_el = ts.ln_lam[TRK_TAB_E_EL];
+        d.ln_rhw_hel = ts.ln_hw[TRK_TAB_H_EL]; d.ln_rlam_hel = ts.ln_lam[TRK_TAB_H_EL];
     }
     d.NR = NR; d.R = s->d_R;
     d.include_photons = h.o.include_photons;
@@ -608,16 +619,18 @@ int run_mc(trk_ctx *c, int64_t nmc, uint64_t seed, int64_t first_iter, const dou
 
     // LDS plan: stage the arrays in order while they fit
     {
-        const int nsz[24] = {d.nE, d.nE, d.nE, d.nE, d.nEel, d.nEel, d.nEel, d.nEel,
-                             d.nH, d.nH, d.nH, d.nH, d.nHel, d.nHel, d.nHel, d.nHel,
-                             NR, Nt, h.m.n_dos, h.m.n_dos, h.m.n_dos, h.m.n_dos, nsh * d.nE, nsh * d.nH};
-        const bool present[24] = {true, true, d.lnE != nullptr, d.ln_el_imfp != nullptr, true, true,
-                                  d.lnEel != nullptr, d.ln_Lel != nullptr, true, true, d.lnEH != nullptr,
-                                  d.ln_hole_imfp != nullptr, true, true, d.lnEHel != nullptr, d.ln_LHel != nullptr,
-                                  true, true, true, true, true, true, true, true};
+        const int nsz[kStaged] = {d.nE, d.nE, d.nE, d.nE, d.nEel, d.nEel, d.nEel, d.nEel,
+                                  d.nH, d.nH, d.nH, d.nH, d.nHel, d.nHel, d.nHel, d.nHel,
+                                  NR, Nt, h.m.n_dos, h.m.n_dos, h.m.n_dos, h.m.n_dos, nsh * d.nE, nsh * d.nH,
+                                  nsh * d.nE, nsh * d.nH};
+        const bool present[kStaged] = {true, true, d.lnE != nullptr, d.ln_el_imfp != nullptr, true, true,
+                                       d.lnEel != nullptr, d.ln_Lel != nullptr, true, true, d.lnEH != nullptr,
+                                       d.ln_hole_imfp != nullptr, true, true, d.lnEHel != nullptr,
+                                       d.ln_LHel != nullptr, true, true, true, true, true, true, true, true,
+                                       d.ln_Lsh != nullptr, d.ln_LHsh != nullptr};
         const size_t limit = (size_t)s->max_lds / sizeof(double);
         size_t off = 0;
-        for (int a = 0; a < 24; ++a) {
+        for (int a = 0; a < kStaged; ++a) {
             size_t n = present[a] ? (size_t)nsz[a] : 0;
             if (off + n > limit) n = 0;
             d.lds_n[a] = (int)n;

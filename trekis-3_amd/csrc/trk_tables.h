@@ -67,6 +67,10 @@ struct TableSet {
     // needs only the log of the particle energy (same values, same result)
     double *ln_E[7] = {};          // per kind, [n]
     double *ln_el_imfp = nullptr, *ln_hole_imfp = nullptr, *ln_Lel = nullptr, *ln_LHel = nullptr;
+    // per kind: logs of the per-shell mean free paths [nsh*n] (inelastic
+    // kinds) and of the cumulative rows' transfer energies and lambdas
+    // (same layout as hw / lam)
+    double *ln_L[7] = {}, *ln_hw[7] = {}, *ln_lam[7] = {};
     int64_t launches = 0;
     double build_seconds = 0.0;
     int64_t dcs_evals = 0;         // number of q-integrals evaluated

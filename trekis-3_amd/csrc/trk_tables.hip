@@ -300,9 +300,9 @@ __global__ void k_inv_row(const double *__restrict__ L, int n, double *__restric
     }
 }
 
-__global__ void k_log(const double *__restrict__ x, int n, double *__restrict__ out)
+__global__ void k_log(const double *__restrict__ x, int64_t n, double *__restrict__ out)
 {
-    int k = blockIdx.x * blockDim.x + threadIdx.x;
+    int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (k < n) out[k] = trk_log(x[k]);
 }
 
@@ -636,6 +636,8 @@ void free_tables(TableSet &ts)
     if (ts.shi_path) (void)hipFree(ts.shi_path);
     ts.el_imfp = ts.hole_imfp = ts.phot_imfp = ts.shi_path = nullptr;
     for (double *&p : ts.ln_E) { if (p) (void)hipFree(p); p = nullptr; }
+    for (int k = 0; k < 7; ++k)
+        for (double **p : {&ts.ln_L[k], &ts.ln_hw[k], &ts.ln_lam[k]}) { if (*p) (void)hipFree(*p); *p = nullptr; }
     double **lp[] = {&ts.ln_el_imfp, &ts.ln_hole_imfp, &ts.ln_Lel, &ts.ln_LHel};
     for (double **p : lp) { if (*p) (void)hipFree(*p); *p = nullptr; }
 }
@@ -836,6 +838,21 @@ int build_tables(const HostMat &h, const MatDev &md, TableSet &ts, hipStream_t s
         if (!g.src || g.n <= 0) continue;
         if (dalloc(g.dst, g.n, err)) return TRK_ERR_DEVICE;
         k_log<<<nblk(g.n, 128), 128, 0, st>>>(g.src, g.n, *g.dst);
+    }
+    // per-shell mean free paths (shell choice) and cumulative rows (transfer
+    // sampling): interpolation then takes one log of the argument
+    for (int k : {TRK_TAB_E_INEL, TRK_TAB_H_INEL}) {
+        const int64_t m = (int64_t)T[k].nsh * T[k].n;
+        if (!T[k].L || m <= 0) continue;
+        if (dalloc(&ts.ln_L[k], m, err)) return TRK_ERR_DEVICE;
+        k_log<<<(unsigned)((m + 127) / 128), 128, 0, st>>>(T[k].L, m, ts.ln_L[k]);
+    }
+    for (int k : {TRK_TAB_E_INEL, TRK_TAB_H_INEL, TRK_TAB_E_EL, TRK_TAB_H_EL}) {
+        const int64_t m = T[k].row_off_h.empty() ? 0 : T[k].row_off_h.back();
+        if (!T[k].hw || m <= 0) continue;
+        if (dalloc(&ts.ln_hw[k], m, err) || dalloc(&ts.ln_lam[k], m, err)) return TRK_ERR_DEVICE;
+        k_log<<<(unsigned)((m + 127) / 128), 128, 0, st>>>(T[k].hw, m, ts.ln_hw[k]);
+        k_log<<<(unsigned)((m + 127) / 128), 128, 0, st>>>(T[k].lam, m, ts.ln_lam[k]);
     }
     HIP_OK(hipGetLastError());
     HIP_OK(hipStreamSynchronize(st));
