@@ -125,7 +125,7 @@ enum { C_E_EL = 0, C_E_INEL, C_H_EL, C_H_INEL, C_DEEP, C_PHOT, C_ION, C_COUNT };
 // their state untouched.  Rare, long branches (ionisation, decays) then run
 // for many lanes at once instead of stalling the wave at almost every step.
 constexpr int kBatchLanes = 16;
-constexpr int kMaxWait = 8;
+constexpr int kMaxWait = 16;
 
 // Events one particle may take in one round.  A particle with more left is
 // written to the next round's list; this bounds the tail of every launch by
