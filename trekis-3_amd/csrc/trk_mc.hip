@@ -76,6 +76,8 @@ struct McDev {
     int nEel; const double *Eel, *Lel; const int64_t *roff_el; const double *rhw_el, *rlam_el;
     int nH; const double *EH, *LHsh, *hole_imfp; const int64_t *roff_h; const double *rhw_h, *rlam_h;
     int nHel; const double *EHel, *LHel; const int64_t *roff_hel; const double *rhw_hel, *rlam_hel;
+    // 1 per cumulative row whose lam never increases (counted search)
+    const unsigned char *rmono, *rmono_h, *rmono_el, *rmono_hel;
     int nP; const double *EP, *LPsh, *phot_imfp;
     int nS; const double *ES, *LSsh, *shi_path;
     const int *sd_n; const int64_t *sd_off; const double *sd_E, *sd_L, *sd_inv; const int *sd_nonmono;
@@ -438,6 +440,8 @@ static void fill_mcdev(const trk_ctx *c, McDev &d)
     d.roff_h = T[TRK_TAB_H_INEL].row_off; d.rhw_h = T[TRK_TAB_H_INEL].hw; d.rlam_h = T[TRK_TAB_H_INEL].lam;
     d.nHel = T[TRK_TAB_H_EL].n; d.EHel = T[TRK_TAB_H_EL].E; d.LHel = T[TRK_TAB_H_EL].L;
     d.roff_hel = T[TRK_TAB_H_EL].row_off; d.rhw_hel = T[TRK_TAB_H_EL].hw; d.rlam_hel = T[TRK_TAB_H_EL].lam;
+    d.rmono = T[TRK_TAB_E_INEL].row_mono; d.rmono_h = T[TRK_TAB_H_INEL].row_mono;
+    d.rmono_el = T[TRK_TAB_E_EL].row_mono; d.rmono_hel = T[TRK_TAB_H_EL].row_mono;
     d.nP = T[TRK_TAB_PHOT].n; d.EP = T[TRK_TAB_PHOT].E; d.LPsh = T[TRK_TAB_PHOT].L; d.phot_imfp = ts.phot_imfp;
     d.nS = T[TRK_TAB_SHI].n; d.ES = T[TRK_TAB_SHI].E; d.LSsh = T[TRK_TAB_SHI].L; d.shi_path = ts.shi_path;
 }

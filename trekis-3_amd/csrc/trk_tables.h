@@ -43,6 +43,7 @@ struct TableDev {
     double *dEdx = nullptr;
     // cumulative rows: row (shell s, energy i) at offset row_off[s*n+i]
     int64_t *row_off = nullptr;   // [nsh*n+1], device
+    unsigned char *row_mono = nullptr;  // [nsh*n], 1 if the row's lam never increases (no NaN)
     double *hw = nullptr;
     double *lam = nullptr;
     std::vector<int64_t> row_off_h;   // host copy of offsets

@@ -306,6 +306,22 @@ __global__ void k_log(const double *__restrict__ x, int64_t n, double *__restric
     if (k < n) out[k] = trk_log(x[k]);
 }
 
+// mono[slot] = 1 if lam of the row never increases (and holds no NaN): the
+// row search can then count entries instead of bisecting
+__global__ void k_row_mono(const int64_t *__restrict__ off, int64_t nslots, const double *__restrict__ lam,
+                           unsigned char *__restrict__ mono)
+{
+    int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= nslots) return;
+    const int64_t a = off[k], b = off[k + 1];
+    unsigned char m = 1;
+    for (int64_t i = a; i < b; ++i) {
+        if (!(lam[i] == lam[i])) m = 0;
+        if (i + 1 < b && !(lam[i] >= lam[i + 1])) m = 0;
+    }
+    mono[k] = m;
+}
+
 struct RowOut {
     double *L, *dEdx;        // table arrays for the row's kind (shell-major)
     int n;                   // table grid size
@@ -624,6 +640,7 @@ void free_tables(TableSet &ts)
         if (t.row_off) (void)hipFree(t.row_off);
         if (t.hw) (void)hipFree(t.hw);
         if (t.lam) (void)hipFree(t.lam);
+        if (t.row_mono) (void)hipFree(t.row_mono);
         if (t.dE) (void)hipFree(t.dE);
         if (t.dL) (void)hipFree(t.dL);
         if (t.dInv) (void)hipFree(t.dInv);
@@ -853,6 +870,9 @@ int build_tables(const HostMat &h, const MatDev &md, TableSet &ts, hipStream_t s
         if (dalloc(&ts.ln_hw[k], m, err) || dalloc(&ts.ln_lam[k], m, err)) return TRK_ERR_DEVICE;
         k_log<<<(unsigned)((m + 127) / 128), 128, 0, st>>>(T[k].hw, m, ts.ln_hw[k]);
         k_log<<<(unsigned)((m + 127) / 128), 128, 0, st>>>(T[k].lam, m, ts.ln_lam[k]);
+        const int64_t nslots = (int64_t)T[k].nsh * T[k].n;
+        if (dalloc(&T[k].row_mono, nslots, err)) return TRK_ERR_DEVICE;
+        k_row_mono<<<(unsigned)((nslots + 127) / 128), 128, 0, st>>>(T[k].row_off, nslots, T[k].lam, T[k].row_mono);
     }
     HIP_OK(hipGetLastError());
     HIP_OK(hipStreamSynchronize(st));
