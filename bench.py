@@ -47,15 +47,15 @@ SNAP_E, SNAP_H = 48, 72
 
 def transport_bytes(ev, nsh, snaps_e, snaps_h, snaps_p, phonon_rows):
     """Algorithmic bytes of the transport kernel's events (the ion kernel's
-    SHI events excluded)."""
+    SHI events excluded): (total, the part without the per-node snapshot
+    terms)."""
     e_inel, e_el, h_inel, h_el, aug, rad, phot = (int(ev[i]) for i in range(1, 8))
     b = (e_inel + e_el) * 2 * STATE_E + (h_inel + h_el + aug + rad) * 2 * STATE_H + phot * 2 * STATE_E
     b += (e_inel + h_inel + aug + phot) * SPAWN_PAIR + rad * SPAWN_PHOT
     g = e_inel * (2 + nsh + 2) + e_el * (2 + (2 if phonon_rows else 0)) + h_inel * (2 + nsh + 2) \
         + h_el * (2 + (2 if phonon_rows else 0)) + (aug + rad) * nsh + phot * (1 + nsh)
     b += g * GATHER
-    b += snaps_e * SNAP_E + snaps_h * SNAP_H + snaps_p * SNAP_E
-    return b
+    return b + snaps_e * SNAP_E + snaps_h * SNAP_H + snaps_p * SNAP_E, b
 
 
 # radial tallies and the particle count that governs each: the error is the
@@ -195,8 +195,9 @@ def main():
     value = total_ev / t_total
     ms_tr, n_tr = main_leg["ms_tr"], main_leg["n_tr"]
     nsh = sum(mat.n_shells)
-    b_tr = transport_bytes(ev, nsh, *[int(round(s)) for s in main_leg["snaps"]], opt.kind_of_EMFP == 1)
+    b_tr, b_ev = transport_bytes(ev, nsh, *[int(round(s)) for s in main_leg["snaps"]], opt.kind_of_EMFP == 1)
     per_launch = b_tr / max(n_tr, 1)
+    per_launch_ev = b_ev / max(n_tr, 1)
     mean_s = (ms_tr / max(n_tr, 1)) * 1e-3
     achieved = per_launch / mean_s / 1e9 if mean_s > 0 else 0.0
 
@@ -272,7 +273,12 @@ def main():
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "kernel": kname, "launches": n_tr, "mean_launch_ms": ms_tr / max(n_tr, 1),
                      "alg_bytes_per_launch": per_launch,
-                     "traffic_GBs": (traffic / mean_s / 1e9) if (traffic and mean_s > 0) else None},
+                     # the model's per-node snapshot bytes are work the difference-array
+                     # tallies never do; without them:
+                     "alg_bytes_per_launch_without_snapshots": per_launch_ev,
+                     "frac_without_snapshots": (per_launch_ev / mean_s / 1e9 / HBM_PEAK_GBS) if mean_s > 0 else None,
+                     "traffic_GBs": (traffic / mean_s / 1e9) if (traffic and mean_s > 0) else None,
+                     "traffic_frac": (traffic / mean_s / 1e9 / HBM_PEAK_GBS) if (traffic and mean_s > 0) else None},
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)
