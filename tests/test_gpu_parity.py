@@ -19,11 +19,13 @@ from gpu_helpers import setup, say, sort_trace, first_mismatch, compare_tallies
 
 pytestmark = pytest.mark.gpu
 
-SMALL = ["test", "test_metal"]
+SMALL = ["test", "test_metal", "test_sio2"]
 FULL = ["config1", "config2"]       # the Al-like/Xe and Al2O3-like/Au workloads at full size
+# table parity: test_sio2 has config5's tables (SiO2-like, U 20 MeV/u, phonon CDF, photons)
+TABLES = ["test", "test_metal"] + FULL + ["config5"]
 
 
-@pytest.mark.parametrize("name", SMALL + FULL)
+@pytest.mark.parametrize("name", TABLES)
 def test_tables_bitwise(name):
     ctx, orc, mat, ion, opt, run, R, V = setup(name, build_oracle_tables=True)
     nsh = sum(mat.n_shells)

@@ -260,13 +260,50 @@ def _al_like(layer=100.0):
                         dos_raw_E=E, dos_raw=D)
 
 
+def _sio2_like(layer=1.0e4, decay_seed=0x5EED0005):
+    """SiO2-like target of the 'large single track' configuration: density
+    2.32 g/cm^3, 9 eV gap, five valence oscillators, Si K / L1 / L2,3 and O K
+    shells, three phonon oscillators with E0 in 0.06-0.15 eV (quasi-elastic
+    scattering on the phonon CDF).  Oscillator positions, widths, ionisation
+    potentials and Auger times are INPUT_CDF/SiO2.cdf's (the valence set
+    without its weakest oscillator); the radiative times are drawn like the
+    Al2O3-like target's, from a branching ratio per core shell."""
+    rng = np.random.default_rng(decay_seed)
+    br = rng.uniform(0.0, 1.0, size=4)
+    def split(tau, b):
+        return tau / (1.0 - b), tau / b
+    aK, rK = split(1.6, br[0])
+    aL1, rL1 = split(16.0, br[1])
+    aL23, rL23 = split(16.0, br[2])
+    aOK, rOK = split(8.0, br[3])
+    si = Atom(14, 28.09, 1.0, [
+        Shell(1844.1, 2.0, [(2060.0, 1.0, 946.5)], auger=aK, radiat=rK, shl_num=1),
+        Shell(151.6, 2.0, [(232.0, 1.0, 124.4)], auger=aL1, radiat=rL1, shl_num=3),
+        Shell(108.0, 6.0, [(149.75, 0.94, 120.0), (120.52, 0.06, 16.52)], auger=aL23, radiat=rL23, shl_num=4),
+        Shell(9.0, 16.0, [(41.385, 0.26389, 0.1856), (23.63, 262.65, 11.79), (14.5927, 5.783, 3.0),
+                          (53.904, 177.4, 32.89), (34.064, 55.788, 14.2)]),
+    ])
+    o = Atom(8, 16.00, 2.0, [
+        Shell(538.25, 2.0, [(481.0, 1.0, 406.5)], auger=aOK, radiat=rOK, shl_num=1),
+    ])
+    E, D = dos_insulator()
+    return MaterialDesc("SiO2-like", [si, o], dens=2.32, layer=layer, vsound=5170.0, E_F=10.0,
+                        el_eff_mass=1.0, hole_mass=1.0, cut_off=10.0,
+                        phonon=[(0.06, 0.0005, 0.009), (0.10, 0.0015, 0.013), (0.15, 0.0004, 0.009)],
+                        dos_raw_E=E, dos_raw=D)
+
+
 def config_material(name):
     """(MaterialDesc, Ion, Options, run dict) for a named configuration.
 
     config1  Al-like metal, Xe 1 MeV/u, 100 iterations, 0-100 fs (dt 0.1)
     config2  Al2O3-like, Au 11 MeV/u, 10^4 iterations, 0-1000 fs (dt 1)
+    config5  SiO2-like with phonon quasi-elastic scattering and photons, U 20
+             MeV/u over a 1 um track, 10^6 iterations, 0-1000 fs, 200 radial bins
     test     config2's target with a reduced electron grid (2 keV), O ion at
              0.5 MeV/u, 0-100 fs: small enough for the CPU oracle
+    test_sio2  config5 on a 100 A track segment, 2 iterations (the full
+             tables; a cascade the CPU oracle replays in seconds)
     """
     opt = abi.Options()
     ion = abi.Ion()
@@ -283,6 +320,13 @@ def config_material(name):
         opt.Tim, opt.dt, opt.dt_flag = 1000.0, 1.0, 0
         opt.include_photons = 1
         run = dict(nmc=10000, NR=100, Rmin=1.0, Rmax=1000.0, seed=0x5EED0002)
+    elif name in ("config5", "test_sio2"):
+        mat = _sio2_like(layer=1.0e4 if name == "config5" else 100.0)
+        ion.Zat, ion.Mass, ion.E = 92, 238.03, 20.0e6 * 238.03
+        opt.kind_of_EMFP, opt.CDF_elast_Zeff = 1, 1
+        opt.Tim, opt.dt, opt.dt_flag = 1000.0, 1.0, 0
+        opt.include_photons = 1
+        run = dict(nmc=10 ** 6 if name == "config5" else 2, NR=200, Rmin=1.0, Rmax=1.0e4, seed=0x5EED0005)
     elif name == "test":
         mat = _al2o3_like()
         ion.Zat, ion.Mass, ion.E = 8, 16.00, 0.5e6 * 16.0
