@@ -25,8 +25,9 @@ Tolerances, per radial tally (time x radius; hole tallies summed over shells):
   - relative L1 difference over all bins and nodes: 5e-3.
 Photon tallies of these workloads hold only a few photon counts at the time
 nodes and are not checked per bin.  The total deposited energy (Out_tot_E)
-agrees to 1e-6 relative: the ion's energy transfers are sampled in FP32 too,
-so a flipped ionisation changes it, by far less than its noise."""
+agrees to 1e-5 relative: the ion's energy transfers are sampled in FP32 and
+its position is kept in FP32 too, so a flipped ionisation changes it, by far
+less than its noise (~1e-3 for these iteration counts)."""
 import numpy as np
 import pytest
 
@@ -93,7 +94,7 @@ def test_mixed_radial_tallies_within_1pct(name):
         assert l1 <= 5e-3, k
     te_a, te_b = float(ref["Out_tot_E"].sum()), float(mix["Out_tot_E"].sum())
     say(f"[mixed] {name} Out_tot_E: {te_a:.12e} vs {te_b:.12e}")
-    assert abs(te_b - te_a) <= 1e-6 * abs(te_a)
+    assert abs(te_b - te_a) <= 1e-5 * abs(te_a)
 
 
 def test_mixed_repeatable():

@@ -136,7 +136,12 @@ constexpr uint32_t kRoundEvents = 512;
 
 extern __shared__ double lds_tab[];
 
-constexpr int kTransportBlock = 768;   // 12 waves: one workgroup per CU at 3 waves per SIMD
+// Transport workgroup of each path: one per CU (the staged tables fill most
+// of the LDS), 12 waves, 3 per SIMD (168 VGPRs).  At 16 waves the mixed
+// path would have to spill to scratch, which costs more than the fourth
+// wave gains.
+constexpr int kTransportBlockFp64 = 768;
+constexpr int kTransportBlockMx = 768;
 
 // Next_free_path_2d (MC:1874-1899) and _1d (MC:1835-1871)
 TRK_HD double nfp_2d(const double *Ea, const double *La, int n, double E)
@@ -647,9 +652,9 @@ int run_mc(trk_ctx *c, int64_t nmc, uint64_t seed, int64_t first_iter, const dou
             HIPC(hipFuncSetAttribute(kt, hipFuncAttributeMaxDynamicSharedMemorySize, (int)s->lds_bytes));
         int per_cu = 0;
         if (mixed)
-            HIPC(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, mx::k_transport, kTransportBlock, s->lds_bytes));
+            HIPC(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, mx::k_transport, kTransportBlockMx, s->lds_bytes));
         else
-            HIPC(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fp64::k_transport, kTransportBlock, s->lds_bytes));
+            HIPC(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fp64::k_transport, kTransportBlockFp64, s->lds_bytes));
         if (per_cu < 1) per_cu = 1;
         s->max_blocks = (int64_t)per_cu * s->ncu;
     }
@@ -717,13 +722,14 @@ int run_mc(trk_ctx *c, int64_t nmc, uint64_t seed, int64_t first_iter, const dou
             HIPC(hipMemsetAsync(s->d_counts + 4 * nxt, 0, 4 * sizeof(int), st));
             for (int k = 0; k < 4; ++k) { d.in[k] = s->L[cur][k]; d.out[k] = s->L[nxt][k]; d.n_in[k] = nin[k]; }
             d.out_cnt = s->d_counts + 4 * nxt;
-            int64_t nblk = (nthr + kTransportBlock - 1) / kTransportBlock;
+            const int tb = mixed ? kTransportBlockMx : kTransportBlockFp64;
+            int64_t nblk = (nthr + tb - 1) / tb;
             if (nblk > s->max_blocks) nblk = s->max_blocks;
             d.cursor = s->d_counts + 8;
             HIPC(hipMemsetAsync(d.cursor, 0, sizeof(int), st));
             if (c->prof) HIPC(hipEventRecord(c->pe[0], st));
-            if (mixed) mx::k_transport<<<(unsigned)nblk, kTransportBlock, s->lds_bytes, st>>>(d);
-            else fp64::k_transport<<<(unsigned)nblk, kTransportBlock, s->lds_bytes, st>>>(d);
+            if (mixed) mx::k_transport<<<(unsigned)nblk, kTransportBlockMx, s->lds_bytes, st>>>(d);
+            else fp64::k_transport<<<(unsigned)nblk, kTransportBlockFp64, s->lds_bytes, st>>>(d);
             HIPC(hipGetLastError());
             if (c->prof) { HIPC(hipEventRecord(c->pe[1], st)); pending = 1; }
             launches++;
