@@ -91,7 +91,7 @@ struct McDev {
     double T_last, Tim, cutoff_eff;
     int Nt; const double *tim_glob;
     double inv_dt;                      // 1/dt of a linear time grid, 0 otherwise
-    int hints;                          // 1: the electron and hole energy grids and the time grid are strictly increasing
+    int hints;                          // 1: the electron and hole energy grids, the time grid and Out_R are strictly increasing
     // tables copied into each transport workgroup's LDS (count and offset in
     // doubles; count 0 = read from global memory), see kStage* below
     int lds_n[kStaged], lds_off[kStaged];
@@ -553,9 +553,11 @@ int run_mc(trk_ctx *c, int64_t nmc, uint64_t seed, int64_t first_iter, const dou
                 if (!(g[i - 1] < g[i])) d.hints = 0;
         }
         // and the time grid (a flight with no node in (t0, tn] is recognised
-        // from the first node after t0)
+        // from the first node after t0) and the radii Out_R
         for (int i = 1; i < Nt; ++i)
             if (!(h.tim_glob[i - 1] < h.tim_glob[i])) d.hints = 0;
+        for (int i = 1; i < NR; ++i)
+            if (!(Out_R[i - 1] < Out_R[i])) d.hints = 0;
     }
     {
         const TableSet &ts = c->ts;
