@@ -94,6 +94,24 @@ def test_mc_batching_and_replicas_exact(name):
     assert ((parts[0].n_events + parts[1].n_events) == base.n_events).all()
 
 
+def test_mc_replay_partly_staged():
+    """With most tables left in HBM (a small LDS budget), the transport kernel
+    that tests each table for staging replays the oracle bit for bit too."""
+    import os
+    ctx, orc, mat, ion, opt, run, R, V = setup("test")
+    os.environ["TRK_LDS_LIMIT_BYTES"] = "16384"
+    try:
+        say("[mc] test: GPU with a 16 KB LDS budget")
+        g = ctx.run_mc(run["nmc"], run["seed"], R, V, trace_cap=3_000_000)
+    finally:
+        del os.environ["TRK_LDS_LIMIT_BYTES"]
+    c = orc.run_mc(run["nmc"], run["seed"], R, V, tg.TallySet(orc.Nt, len(R), mat), trace_cap=3_000_000)
+    assert g.trace_total == len(c.trace)
+    mm = first_mismatch(sort_trace(g.trace), sort_trace(c.trace))
+    assert mm is None, mm
+    assert not compare_tallies(g, c)
+
+
 def test_error_paths():
     lib = tg.load_library()
     mat, ion, opt, run = tg.config_material("test")

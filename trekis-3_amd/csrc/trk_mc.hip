@@ -316,6 +316,7 @@ struct McState {
     int64_t max_blocks = 0;   // resident transport workgroups on the device
     int ncu = 0, max_lds = 0;
     size_t lds_bytes = 0;     // staged tables per transport workgroup
+    bool all_staged = false;  // every staged array fits (the k_transport<true> instantiation runs)
     int *d_overflow = nullptr;
     unsigned long long *d_counters = nullptr;
     Tally T{};
@@ -639,24 +640,31 @@ int run_mc(trk_ctx *c, int64_t nmc, uint64_t seed, int64_t first_iter, const dou
                                        d.ln_hole_imfp != nullptr, true, true, d.lnEHel != nullptr,
                                        d.ln_LHel != nullptr, true, true, true, true, true, true, true, true,
                                        d.ln_Lsh != nullptr, d.ln_LHsh != nullptr};
-        const size_t limit = (size_t)s->max_lds / sizeof(double);
+        size_t limit = (size_t)s->max_lds / sizeof(double);
+        // TRK_LDS_LIMIT_BYTES lowers the budget (tests of the partly staged kernel)
+        if (const char *e = getenv("TRK_LDS_LIMIT_BYTES")) {
+            const long long v = atoll(e);
+            if (v >= 0 && (size_t)v / sizeof(double) < limit) limit = (size_t)v / sizeof(double);
+        }
         size_t off = 0;
+        bool all = true;
         for (int a = 0; a < kStaged; ++a) {
             size_t n = present[a] ? (size_t)nsz[a] : 0;
             if (off + n > limit) n = 0;
+            if (!present[a] || n != (size_t)nsz[a] || n == 0) all = false;
             d.lds_n[a] = (int)n;
             d.lds_off[a] = (int)off;
             off += n;
         }
         s->lds_bytes = off * sizeof(double);
-        const void *kt = mixed ? (const void *)mx::k_transport : (const void *)fp64::k_transport;
+        s->all_staged = all;
+        const void *kt = mixed ? (all ? (const void *)mx::k_transport<true> : (const void *)mx::k_transport<false>)
+                               : (all ? (const void *)fp64::k_transport<true> : (const void *)fp64::k_transport<false>);
         if (s->lds_bytes > 65536)
             HIPC(hipFuncSetAttribute(kt, hipFuncAttributeMaxDynamicSharedMemorySize, (int)s->lds_bytes));
         int per_cu = 0;
-        if (mixed)
-            HIPC(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, mx::k_transport, kTransportBlockMx, s->lds_bytes));
-        else
-            HIPC(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fp64::k_transport, kTransportBlockFp64, s->lds_bytes));
+        HIPC(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kt, mixed ? kTransportBlockMx : kTransportBlockFp64,
+                                                          s->lds_bytes));
         if (per_cu < 1) per_cu = 1;
         s->max_blocks = (int64_t)per_cu * s->ncu;
     }
@@ -730,8 +738,10 @@ int run_mc(trk_ctx *c, int64_t nmc, uint64_t seed, int64_t first_iter, const dou
             d.cursor = s->d_counts + 8;
             HIPC(hipMemsetAsync(d.cursor, 0, sizeof(int), st));
             if (c->prof) HIPC(hipEventRecord(c->pe[0], st));
-            if (mixed) mx::k_transport<<<(unsigned)nblk, kTransportBlockMx, s->lds_bytes, st>>>(d);
-            else fp64::k_transport<<<(unsigned)nblk, kTransportBlockFp64, s->lds_bytes, st>>>(d);
+            if (mixed && s->all_staged) mx::k_transport<true><<<(unsigned)nblk, kTransportBlockMx, s->lds_bytes, st>>>(d);
+            else if (mixed) mx::k_transport<false><<<(unsigned)nblk, kTransportBlockMx, s->lds_bytes, st>>>(d);
+            else if (s->all_staged) fp64::k_transport<true><<<(unsigned)nblk, kTransportBlockFp64, s->lds_bytes, st>>>(d);
+            else fp64::k_transport<false><<<(unsigned)nblk, kTransportBlockFp64, s->lds_bytes, st>>>(d);
             HIPC(hipGetLastError());
             if (c->prof) { HIPC(hipEventRecord(c->pe[1], st)); pending = 1; }
             launches++;
