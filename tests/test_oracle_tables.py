@@ -66,6 +66,38 @@ def test_find_decreasing_and_linear(oracle_lib, N):
         assert oracle_lib.or_find_linear(dp(I), N, float(v)) == min(lin, N)
 
 
+def test_find_decr_is_a_count_on_nonincreasing_rows(oracle_lib):
+    """The transport kernel replaces the reference's bisection over a
+    cumulative row whose entries never increase by a count: the bisection's
+    result is max(1, number of entries > v) whatever probe path it takes,
+    plateaus and node values included.  (A NaN value, which the oracle
+    reports as an error, the device search maps to 1, as the count does.)"""
+    r = np.random.default_rng(7)
+    for trial in range(400):
+        N = int(r.integers(1, 300))
+        D = np.sort(r.integers(0, int(r.choice([3, 40, 1000])), N).astype(np.float64))[::-1].copy()
+        vals = np.concatenate([r.uniform(D[-1] - 2, D[0] + 2, 20), D[r.integers(0, N, 5)]])
+        for v in vals:
+            exp = max(1, int(np.sum(D > v)))
+            assert oracle_lib.or_find_decr(dp(D), N, float(v)) == exp, (trial, N, v)
+
+
+def test_bracket_guess_equals_search(oracle_lib):
+    """A particle's last bracket h is taken without a search only when its
+    value lies strictly inside [A(h-1), A(h)] of a strictly increasing grid;
+    there both bisection variants return h."""
+    r = np.random.default_rng(11)
+    for N in (2, 3, 9, 200, 581):
+        A = np.cumsum(r.uniform(0.01, 3.0, N))
+        for h in range(2, N + 1):
+            lo, hi = A[h - 2], A[h - 1]
+            for v in (lo + (hi - lo) * f for f in (1e-9, 0.37, 1 - 1e-9)):
+                if not (lo < v < hi):
+                    continue
+                assert oracle_lib.or_find_mono_2d(dp(A), N, float(v)) == h
+                assert oracle_lib.or_find_mono_1d(dp(A), N, float(v)) == h
+
+
 # ---------------------------------------------------------- interpolation
 @pytest.mark.parametrize("flag,f", [
     (1, lambda E: 3.0 - 0.25 * E),
