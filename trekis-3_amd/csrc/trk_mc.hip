@@ -202,9 +202,11 @@ __global__ void k_prefix_fx(unsigned long long *a, int B, int Nt)
 __global__ void k_norm_hist(double *acc, int accld, const int *cnt, const int *norm, int B, int Nt, int ncol, int wmode,
                             const double *grid)
 {
+    // neighbouring threads take neighbouring columns j of the same node i:
+    // the per-iteration rows cnt[b][i][:] are read coalesced
     int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= (int64_t)Nt * ncol) return;
-    int i = (int)(t % Nt), j = (int)(t / Nt);
+    int j = (int)(t % ncol), i = (int)(t / ncol);
     double w = 1.0;
     if (wmode == 1) w = (j > 0) ? 1.0 / (grid[j] - grid[j - 1]) : 1.0 / grid[0];
     else if (wmode == 2) w = (j > 0) ? 1.0 / (grid[j] - grid[j - 1]) : 1.0 / (grid[1] - grid[0]);
@@ -223,13 +225,27 @@ __global__ void k_diff_coeff(double *D, const unsigned long long *dcs, const int
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= Nt) return;
     double v = D[i];
-    for (int b = 0; b < B; ++b) {
-        int n = dcn[(size_t)b * Nt + i];
-        if (n > 0) {
-            long long x = (long long)dcs[((size_t)b * Nt + i) * 2];
-            long long hi = x >> 40;
-            double s = (double)hi + (double)(x - (hi << 40)) * 0x1.0p-40;
-            v = (v + s) / (double)n;
+    // the recurrence runs over the iterations in order; their counts and
+    // sums are loaded sixteen at a time ahead of it
+    for (int b0 = 0; b0 < B; b0 += 16) {
+        int n[16];
+        long long x[16];
+#pragma unroll
+        for (int k = 0; k < 16; ++k) {
+            n[k] = 0;
+            x[k] = 0;
+            if (b0 + k < B) {
+                n[k] = dcn[(size_t)(b0 + k) * Nt + i];
+                x[k] = (long long)dcs[((size_t)(b0 + k) * Nt + i) * 2];
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < 16; ++k) {
+            if (n[k] > 0) {
+                long long hi = x[k] >> 40;
+                double s = (double)hi + (double)(x[k] - (hi << 40)) * 0x1.0p-40;
+                v = (v + s) / (double)n[k];
+            }
         }
     }
     D[i] = v;
@@ -239,9 +255,9 @@ __global__ void k_diff_coeff(double *D, const unsigned long long *dcs, const int
 // Sums the R replicas of every tally cell and takes the prefix over time of
 // the difference arrays, in exact integer arithmetic, leaving the result in
 // replica 0: counts as int64, energies as int128 (low word, high word) in
-// units of 2^-40 eV.  One thread per column: electrons and photons by radial
-// bin, the lattice energy by radial bin (per node, no prefix), holes by
-// (radial bin, shell).
+// units of 2^-40 eV.  Two kernels: the replica sum per cell, then the running
+// sum per column (electrons and photons by radial bin, holes by radial bin
+// and shell; the lattice energy per node has no prefix).
 TRK_DEV __int128 fx_read(const unsigned long long *p)
 {
     return (__int128)(long long)p[1] * ((__int128)1 << 32) + (__int128)p[0];
@@ -253,54 +269,81 @@ TRK_DEV void i128_store(unsigned long long *p, __int128 v)
     p[1] = (unsigned long long)(v >> 64);
 }
 
-__global__ void k_tally_reduce(Tally T)
+// (1) one thread per cell: the replicas summed into replica 0 (counts as
+// int64, energies as int128)
+__global__ void k_replica_sum(Tally T)
 {
     const int NR = T.NR, Nt = T.Nt, nsh = T.nsh, R = T.R;
+    const size_t plane = (size_t)Nt * NR, hplane = plane * nsh;
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const size_t plane = (size_t)Nt * NR;
-    if (t < 2 * NR) {                       // electrons, photons
+    if (t < 2 * (int64_t)plane) {                       // electrons, photons
+        const size_t a = (size_t)(t < (int64_t)plane ? t : t - plane);
+        unsigned long long *cnt = (t < (int64_t)plane) ? T.cnt_e : T.cnt_p;
+        unsigned long long *en = (t < (int64_t)plane) ? T.E_e : T.E_p;
+        long long sc = 0;
+        __int128 se = 0;
+        for (int r = 0; r < R; ++r) {
+            sc += (long long)cnt[r * plane + a];
+            se += fx_read(en + 2 * (r * plane + a));
+        }
+        cnt[a] = (unsigned long long)sc;
+        i128_store(en + 2 * a, se);
+    } else if (t < 3 * (int64_t)plane) {                // lattice energy
+        const size_t a = (size_t)(t - 2 * plane);
+        __int128 se = 0;
+        for (int r = 0; r < R; ++r) se += fx_read(T.Elat + 2 * (r * plane + a));
+        i128_store(T.Elat + 2 * a, se);
+    } else if (t < 3 * (int64_t)plane + (int64_t)hplane) {   // holes
+        const size_t a = (size_t)(t - 3 * plane);
+        long long sc = 0;
+        __int128 sE = 0, sK = 0;
+        for (int r = 0; r < R; ++r) {
+            sc += (long long)T.cnt_h[r * hplane + a];
+            sE += fx_read(T.E_h + 2 * (r * hplane + a));
+            sK += fx_read(T.Ek_h + 2 * (r * hplane + a));
+        }
+        T.cnt_h[a] = (unsigned long long)sc;
+        i128_store(T.E_h + 2 * a, sE);
+        i128_store(T.Ek_h + 2 * a, sK);
+    }
+}
+
+TRK_DEV __int128 i128_load(const unsigned long long *p)
+{
+    return (__int128)(((unsigned __int128)p[1] << 64) | (unsigned __int128)p[0]);
+}
+
+// (2) one thread per column: the running sum over the time nodes of the
+// difference arrays in replica 0 (the lattice energy is not differenced)
+__global__ void k_time_prefix(Tally T)
+{
+    const int NR = T.NR, Nt = T.Nt, nsh = T.nsh;
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < 2 * NR) {
         const int j = (int)(t % NR);
         unsigned long long *cnt = (t < NR) ? T.cnt_e : T.cnt_p;
         unsigned long long *en = (t < NR) ? T.E_e : T.E_p;
         long long rc = 0;
         __int128 re = 0;
+#pragma unroll 8
         for (int i = 0; i < Nt; ++i) {
             const size_t a = (size_t)i * NR + j;
-            long long sc = 0;
-            __int128 se = 0;
-            for (int r = 0; r < R; ++r) {
-                sc += (long long)cnt[r * plane + a];
-                se += fx_read(en + 2 * (r * plane + a));
-            }
-            rc += sc;
-            re += se;
+            rc += (long long)cnt[a];
+            re += i128_load(en + 2 * a);
             cnt[a] = (unsigned long long)rc;
             i128_store(en + 2 * a, re);
         }
-    } else if (t < 3 * NR) {                // lattice energy per node
-        const int j = (int)(t - 2 * NR);
-        for (int i = 0; i < Nt; ++i) {
-            const size_t a = (size_t)i * NR + j;
-            __int128 se = 0;
-            for (int r = 0; r < R; ++r) se += fx_read(T.Elat + 2 * (r * plane + a));
-            i128_store(T.Elat + 2 * a, se);
-        }
-    } else if (t < 3 * NR + (int64_t)NR * nsh) {   // holes
-        const int64_t q = t - 3 * NR;
+    } else if (t < 2 * NR + (int64_t)NR * nsh) {
+        const int64_t q = t - 2 * NR;
         const int j = (int)(q / nsh), sh = (int)(q % nsh);
-        const size_t hplane = plane * nsh;
         long long rc = 0;
         __int128 rE = 0, rK = 0;
+#pragma unroll 8
         for (int i = 0; i < Nt; ++i) {
             const size_t a = ((size_t)i * NR + j) * nsh + sh;
-            long long sc = 0;
-            __int128 sE = 0, sK = 0;
-            for (int r = 0; r < R; ++r) {
-                sc += (long long)T.cnt_h[r * hplane + a];
-                sE += fx_read(T.E_h + 2 * (r * hplane + a));
-                sK += fx_read(T.Ek_h + 2 * (r * hplane + a));
-            }
-            rc += sc; rE += sE; rK += sK;
+            rc += (long long)T.cnt_h[a];
+            rE += i128_load(T.E_h + 2 * a);
+            rK += i128_load(T.Ek_h + 2 * a);
             T.cnt_h[a] = (unsigned long long)rc;
             i128_store(T.E_h + 2 * a, rE);
             i128_store(T.Ek_h + 2 * a, rK);
@@ -792,8 +835,10 @@ int run_mc(trk_ctx *c, int64_t nmc, uint64_t seed, int64_t first_iter, const dou
     // finalize: replicas summed and time prefix taken on the device (exact
     // integers); the host converts, scales by Out_V and forms the totals
     {
-        const int64_t ncol = 3 * (int64_t)NR + (int64_t)NR * nsh;
-        k_tally_reduce<<<(unsigned)((ncol + 127) / 128), 128, 0, st>>>(T);
+        const int64_t ncell = (3 + (int64_t)nsh) * (int64_t)Nt * NR;
+        k_replica_sum<<<(unsigned)((ncell + 255) / 256), 256, 0, st>>>(T);
+        const int64_t ncol = 2 * (int64_t)NR + (int64_t)NR * nsh;
+        k_time_prefix<<<(unsigned)((ncol + 63) / 64), 64, 0, st>>>(T);
         HIPC(hipGetLastError());
     }
     const size_t plane = (size_t)Nt * NR;
