@@ -211,9 +211,21 @@ __global__ void k_norm_hist(double *acc, int accld, const int *cnt, const int *n
     if (wmode == 1) w = (j > 0) ? 1.0 / (grid[j] - grid[j - 1]) : 1.0 / grid[0];
     else if (wmode == 2) w = (j > 0) ? 1.0 / (grid[j] - grid[j - 1]) : 1.0 / (grid[1] - grid[0]);
     double sum = acc[(size_t)i + (size_t)accld * j];
-    for (int b = 0; b < B; ++b) {
-        int c = cnt[((size_t)b * Nt + i) * ncol + j];
-        if (c) sum = sum + (double)c * (w / (double)norm[(size_t)b * Nt + i]);
+    // iterations in order; their counts loaded eight at a time ahead of the sum
+    for (int b0 = 0; b0 < B; b0 += 8) {
+        int c[8], n[8];
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            c[k] = 0;
+            n[k] = 1;
+            if (b0 + k < B) {
+                c[k] = cnt[((size_t)(b0 + k) * Nt + i) * ncol + j];
+                n[k] = norm[(size_t)(b0 + k) * Nt + i];
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < 8; ++k)
+            if (c[k]) sum = sum + (double)c[k] * (w / (double)n[k]);
     }
     acc[(size_t)i + (size_t)accld * j] = sum;
 }
