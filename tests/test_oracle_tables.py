@@ -82,6 +82,49 @@ def test_find_decr_is_a_count_on_nonincreasing_rows(oracle_lib):
             assert oracle_lib.or_find_decr(dp(D), N, float(v)) == exp, (trial, N, v)
 
 
+def _count_lt_16way(A, v):
+    """The probe schedule of the transport kernel's count of entries below v
+    on a non-decreasing row: 15 probes per round narrow [lo, hi), then up to
+    16 entries are counted one by one."""
+    lo, hi = 0, len(A)
+    guard = 0
+    while guard < 8 and hi - lo > 16:
+        n = hi - lo
+        pos = [lo + ((j + 1) * n) // 16 for j in range(15)]
+        lt = [A[p] < v for p in pos]
+        nlo, nhi = lo, hi
+        for j in range(15):
+            if lt[j]:
+                nlo = pos[j] + 1
+        for j in range(14, -1, -1):
+            if not lt[j]:
+                nhi = pos[j]
+        lo, hi = nlo, nhi
+        guard += 1
+    return lo + sum(1 for k in range(16) if lo + k < hi and A[lo + k] < v)
+
+
+def test_first_at_least_is_a_count_on_nondecreasing_rows():
+    """SHI_energy_transfer takes the first i (1-based, i < N) with
+    1/L(i) >= Tot by a linear scan, or N (MC:1719-1780, trk_oracle_mc.c
+    shi_transfer_rn).  On an inverse row that never decreases and holds no
+    NaN that index is 1 + the number of entries below Tot among the first
+    N-1, plateaus and node values included; the kernel's 16-way narrowing
+    finds that count."""
+    r = np.random.default_rng(13)
+    for trial in range(300):
+        N = int(r.integers(1, 3000)) if trial % 3 else int(r.integers(1, 40))
+        inv = np.sort(r.integers(0, int(r.choice([3, 50, 10 ** 6])), N).astype(np.float64))
+        vals = np.concatenate([r.uniform(inv[0] - 2, inv[-1] + 2, 12), inv[r.integers(0, N, 4)], [np.nan]])
+        for Tot in vals:
+            i = 1
+            while i < N and inv[i - 1] < Tot:
+                i += 1
+            cnt = int(np.sum(inv[:N - 1] < Tot))
+            assert i == cnt + 1, (trial, N, Tot)
+            assert _count_lt_16way(inv[:N - 1], Tot) == cnt, (trial, N, Tot)
+
+
 def test_bracket_guess_equals_search(oracle_lib):
     """A particle's last bracket h is taken without a search only when its
     value lies strictly inside [A(h-1), A(h)] of a strictly increasing grid;

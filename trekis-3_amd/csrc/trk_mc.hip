@@ -108,6 +108,10 @@ struct McDev {
     double *trace; unsigned long long *trace_n; long long trace_cap;
     Tally T;
     Spec S;
+    // per-shell constants of the mixed path's shi_transfer (k_shi_consts);
+    // last, so that the offsets of the fields above stay those the FP64
+    // kernel was tuned with
+    const int *sd_M; const double *sd_c;
 };
 
 enum { EV_SHI = 0, EV_E_INEL, EV_E_EL, EV_H_INEL, EV_H_EL, EV_AUGER, EV_RAD, EV_PHOT };
@@ -578,16 +582,26 @@ int run_mc(trk_ctx *c, int64_t nmc, uint64_t seed, int64_t first_iter, const dou
 
     // SHI_DIFF per-shell layout
     const TableDev &td = c->ts.tab[TRK_TAB_SHI_DIFF];
-    int *d_sdn = nullptr;
+    int *d_sdn = nullptr, *d_sdM = nullptr;
     int64_t *d_sdoff = nullptr;
+    double *d_sdc = nullptr;
     HIPC(hipMalloc((void **)&d_sdn, sizeof(int) * nsh));
     HIPC(hipMalloc((void **)&d_sdoff, sizeof(int64_t) * (nsh + 1)));
+    HIPC(hipMalloc((void **)&d_sdM, sizeof(int) * 2 * nsh));
+    HIPC(hipMalloc((void **)&d_sdc, sizeof(double) * 2 * nsh));
     HIPC(hipMemcpyAsync(d_sdn, td.dn.data(), sizeof(int) * nsh, hipMemcpyHostToDevice, st));
     HIPC(hipMemcpyAsync(d_sdoff, td.doff.data(), sizeof(int64_t) * (nsh + 1), hipMemcpyHostToDevice, st));
 
     McDev d{};
     fill_mcdev(c, d);
     d.sd_n = d_sdn; d.sd_off = d_sdoff; d.sd_E = td.dE; d.sd_L = td.dL; d.sd_inv = td.dInv; d.sd_nonmono = td.dNonMono;
+    // per-shell constants of the mixed path's shi_transfer, computed with its
+    // own arithmetic (the FP64 path evaluates them in every event)
+    if (mixed) {
+        mx::k_shi_consts<<<(nsh + 63) / 64, 64, 0, st>>>(d, nsh, d_sdM, d_sdc);
+        HIPC(hipGetLastError());
+    }
+    d.sd_M = d_sdM; d.sd_c = d_sdc;
     d.seed = seed;
     d.Tim = h.o.Tim;
     d.T_last = h.tim_glob[Nt - 1];
@@ -598,7 +612,7 @@ int run_mc(trk_ctx *c, int64_t nmc, uint64_t seed, int64_t first_iter, const dou
         // a particle's last bracket is tried before a search only on strictly
         // increasing grids, where it gives the search's own result
         d.hints = 1;
-        const int kinds[4] = {TRK_TAB_E_INEL, TRK_TAB_E_EL, TRK_TAB_H_INEL, TRK_TAB_H_EL};
+        const int kinds[5] = {TRK_TAB_E_INEL, TRK_TAB_E_EL, TRK_TAB_H_INEL, TRK_TAB_H_EL, TRK_TAB_SHI};
         for (int k : kinds) {
             const TableDev &t = c->ts.tab[k];
             if (t.n <= 0 || !t.E) continue;
@@ -830,6 +844,8 @@ int run_mc(trk_ctx *c, int64_t nmc, uint64_t seed, int64_t first_iter, const dou
     HIPC(hipStreamSynchronize(st));
     (void)hipFree(d_sdn);
     (void)hipFree(d_sdoff);
+    (void)hipFree(d_sdM);
+    (void)hipFree(d_sdc);
 
     // counters and trace
     unsigned long long ctr[16];
