@@ -108,10 +108,12 @@ struct McDev {
     double *trace; unsigned long long *trace_n; long long trace_cap;
     Tally T;
     Spec S;
-    // per-shell constants of the mixed path's shi_transfer (k_shi_consts);
-    // last, so that the offsets of the fields above stay those the FP64
-    // kernel was tuned with
+    // used by the mixed path only, and last, so that the offsets of the
+    // fields above stay those the FP64 kernel was tuned with: per-shell
+    // constants of shi_transfer (k_shi_consts), logs of the SHI grid, of
+    // its per-shell paths and of the fused path
     const int *sd_M; const double *sd_c;
+    const double *lnES, *ln_LSsh, *ln_shi_path;
 };
 
 enum { EV_SHI = 0, EV_E_INEL, EV_E_EL, EV_H_INEL, EV_H_EL, EV_AUGER, EV_RAD, EV_PHOT };
@@ -640,6 +642,7 @@ int run_mc(trk_ctx *c, int64_t nmc, uint64_t seed, int64_t first_iter, const dou
         d.ln_rhw_h = ts.ln_hw[TRK_TAB_H_INEL]; d.ln_rlam_h = ts.ln_lam[TRK_TAB_H_INEL];
         d.ln_rhw_el = ts.ln_hw[TRK_TAB_E_EL]; d.ln_rlam_el = ts.ln_lam[TRK_TAB_E_EL];
         d.ln_rhw_hel = ts.ln_hw[TRK_TAB_H_EL]; d.ln_rlam_hel = ts.ln_lam[TRK_TAB_H_EL];
+        d.lnES = ts.ln_E[TRK_TAB_SHI]; d.ln_LSsh = ts.ln_L[TRK_TAB_SHI]; d.ln_shi_path = ts.ln_shi_path;
     }
     d.NR = NR; d.R = s->d_R;
     d.include_photons = h.o.include_photons;

@@ -68,6 +68,7 @@ struct TableSet {
     // needs only the log of the particle energy (same values, same result)
     double *ln_E[7] = {};          // per kind, [n]
     double *ln_el_imfp = nullptr, *ln_hole_imfp = nullptr, *ln_Lel = nullptr, *ln_LHel = nullptr;
+    double *ln_shi_path = nullptr;  // log of the ion's fused path
     // per kind: logs of the per-shell mean free paths [nsh*n] (inelastic
     // kinds) and of the cumulative rows' transfer energies and lambdas
     // (same layout as hw / lam)

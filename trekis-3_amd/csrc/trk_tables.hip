@@ -655,7 +655,7 @@ void free_tables(TableSet &ts)
     for (double *&p : ts.ln_E) { if (p) (void)hipFree(p); p = nullptr; }
     for (int k = 0; k < 7; ++k)
         for (double **p : {&ts.ln_L[k], &ts.ln_hw[k], &ts.ln_lam[k]}) { if (*p) (void)hipFree(*p); *p = nullptr; }
-    double **lp[] = {&ts.ln_el_imfp, &ts.ln_hole_imfp, &ts.ln_Lel, &ts.ln_LHel};
+    double **lp[] = {&ts.ln_el_imfp, &ts.ln_hole_imfp, &ts.ln_Lel, &ts.ln_LHel, &ts.ln_shi_path};
     for (double **p : lp) { if (*p) (void)hipFree(*p); *p = nullptr; }
 }
 
@@ -850,7 +850,8 @@ int build_tables(const HostMat &h, const MatDev &md, TableSet &ts, hipStream_t s
     }
     struct { const double *src; int n; double **dst; } lg[] = {
         {ts.el_imfp, T[TRK_TAB_E_INEL].n, &ts.ln_el_imfp}, {ts.hole_imfp, T[TRK_TAB_H_INEL].n, &ts.ln_hole_imfp},
-        {T[TRK_TAB_E_EL].L, T[TRK_TAB_E_EL].n, &ts.ln_Lel}, {T[TRK_TAB_H_EL].L, T[TRK_TAB_H_EL].n, &ts.ln_LHel}};
+        {T[TRK_TAB_E_EL].L, T[TRK_TAB_E_EL].n, &ts.ln_Lel}, {T[TRK_TAB_H_EL].L, T[TRK_TAB_H_EL].n, &ts.ln_LHel},
+        {ts.shi_path, T[TRK_TAB_SHI].n, &ts.ln_shi_path}};
     for (auto &g : lg) {
         if (!g.src || g.n <= 0) continue;
         if (dalloc(g.dst, g.n, err)) return TRK_ERR_DEVICE;
@@ -858,7 +859,7 @@ int build_tables(const HostMat &h, const MatDev &md, TableSet &ts, hipStream_t s
     }
     // per-shell mean free paths (shell choice) and cumulative rows (transfer
     // sampling): interpolation then takes one log of the argument
-    for (int k : {TRK_TAB_E_INEL, TRK_TAB_H_INEL}) {
+    for (int k : {TRK_TAB_E_INEL, TRK_TAB_H_INEL, TRK_TAB_SHI}) {
         const int64_t m = (int64_t)T[k].nsh * T[k].n;
         if (!T[k].L || m <= 0) continue;
         if (dalloc(&ts.ln_L[k], m, err)) return TRK_ERR_DEVICE;
