@@ -1,0 +1,152 @@
+"""Readers of the reference's material files (trekis_gpu.inputs): the
+full-CDF material file (reading_material_parameters, Reading_files_and_
+parameters.f90:1185-1644) and the DOS file (reading_material_DOS, RF:2340-
+2450).  The format rules are pinned on small texts written here; the
+shipped input folder of the reference, where present, is read as a whole."""
+import glob
+import math
+import os
+
+import numpy as np
+import pytest
+
+from trekis_gpu import configs, inputs
+
+REF_CDF = "/root/reference/INPUT_CDF"
+
+TWO_ELEMENTS = """Test oxide   ! name
+2		! elements
+13	2	! Z, contribution
+8	3
+3.95	6510.0	4.4	8.8	! density, sound, Fermi, gap
+2		! shells of Al
+2	2	80.0d0	8	27.27d0	! ncdf, designator, Ip, Nel, Auger
+90.0	369.36	40.0
+130.0	221.62	60.0
+1	63	8.8D0	24	1.0d23
+14.0	115.55	3.0
+1
+1	1	538.0e0	2	4.613e0
+560.0	166.65	120.0
+2		! phonon peaks
+0.09	0.002	0.006
+0.06	0.00005	0.002
+"""
+
+
+def test_full_cdf_file_fields():
+    f = inputs.parse_cdf(TWO_ELEMENTS)
+    assert f.name == "Test oxide"
+    assert [(a["Z"], a["pers"], len(a["shells"])) for a in f.atoms] == [(13, 2.0, 2), (8, 3.0, 1)]
+    assert (f.dens, f.vsound, f.E_F, f.egap) == (3.95, 6510.0, 4.4, 8.8)
+    s = f.atoms[0]["shells"][0]
+    assert (s["ncdf"], s["shl_num"], s["Ip"], s["Nel"], s["auger"]) == (2, 2, 80.0, 8.0, 27.27)
+    assert s["osc"] == [(90.0, 369.36, 40.0), (130.0, 221.62, 60.0)]
+    assert f.atoms[0]["shells"][1]["Ip"] == 8.8 and f.atoms[0]["shells"][1]["shl_num"] == 63
+    assert f.phonon == [(0.09, 0.002, 0.006), (0.06, 0.00005, 0.002)]
+
+
+def test_density_line_with_three_values_is_a_metal():
+    """Four values are read, else three and gap 0 (RF:1326-1333): a '!'
+    after three values is read as the fourth and fails, so the gap is the
+    0.1 eV floor -- as for the reference's own files written that way."""
+    t = TWO_ELEMENTS.replace("3.95	6510.0	4.4	8.8	! density, sound, Fermi, gap",
+                             "3.95	6510.0	4.4	! density, sound, Fermi")
+    assert inputs.parse_cdf(t).egap == 0.1
+    t2 = TWO_ELEMENTS.replace("3.95	6510.0	4.4	8.8	!", "3.95	6510.0	4.4	0.05	!")
+    assert inputs.parse_cdf(t2).egap == 0.1
+
+
+def test_ionisation_potential_floor_and_no_phonon_block():
+    t = TWO_ELEMENTS.replace("1	63	8.8D0	24	1.0d23", "1	63	0.0d0	24	1.0d23")
+    t = t[:t.index("2		! phonon peaks")]
+    f = inputs.parse_cdf(t)
+    assert f.atoms[0]["shells"][1]["Ip"] == 0.1          # RF:1554
+    assert f.phonon is None                               # end of file: single-pole phonon CDF (RF:1590-1593)
+
+
+@pytest.mark.parametrize("edit,msg", [
+    (("2		! elements\n13	2	! Z, contribution\n8	3\n", "Al2O3\n"), "EADL"),
+    (("2	2	80.0d0	8	27.27d0", "0	2	80.0d0	8	27.27d0"), "BEB"),
+    (("2	2	80.0d0	8	27.27d0", "2	-2	80.0d0	8	27.27d0"), "BEB"),
+])
+def test_forms_needing_other_inputs_are_rejected(edit, msg):
+    with pytest.raises(ValueError, match=msg):
+        inputs.parse_cdf(TWO_ELEMENTS.replace(*edit))
+
+
+def test_records_span_blank_lines_and_bad_values_stop():
+    """List-directed reads take values from the following records when a
+    record runs short (blank lines hold none); a '!' where a value is
+    expected stops the reference's reader and this one."""
+    t = TWO_ELEMENTS.replace("130.0	221.62	60.0\n", "130.0	221.62	60.0\n\n\n")
+    assert inputs.parse_cdf(t).atoms[0]["shells"][1]["osc"] == [(14.0, 115.55, 3.0)]
+    t = TWO_ELEMENTS.replace("90.0	369.36	40.0", "90.0\n369.36	40.0")
+    assert inputs.parse_cdf(t).atoms[0]["shells"][0]["osc"][0] == (90.0, 369.36, 40.0)
+    with pytest.raises(ValueError, match="unreadable.*line 8"):
+        inputs.parse_cdf(TWO_ELEMENTS.replace("90.0	369.36	40.0", "90.0	369.36	! no width"))
+    with pytest.raises(ValueError, match="unreadable"):
+        inputs.parse_cdf(TWO_ELEMENTS.replace("3.95	6510.0	4.4	8.8	!", "3.95	6510.0	!"))
+
+
+def test_single_pole_marker_file_is_rejected():
+    t = "Metal\n1\n13 1\n2.7 5000.0 11.17\nVALENCE\n1 63 0 0 0\n"
+    with pytest.raises(ValueError, match="single-pole"):
+        inputs.parse_cdf(t)
+
+
+def test_read_cdf_builds_the_material():
+    """Masses from the element table, the atomic density of RF:1338, the
+    valence band as the first element's last shell, oscillators as given."""
+    E, D = configs.dos_insulator()
+    m = inputs.read_cdf(TWO_ELEMENTS, layer=100.0, dos=(E, D))
+    sm = 2 * 26.9815386 + 3 * 15.9992
+    assert m.At_Dens == 1.0e-3 * 3.95 / (configs.G_MP * sm / 5.0)
+    assert m.N_VB_el == 24.0 and not m.gap_differs
+    assert all(not s.normalize for a in m.atoms for s in a.shells)
+    assert m.atoms[1].shells[0].osc == [(560.0, 166.65, 120.0)]
+    assert m.phonon == [(0.09, 0.002, 0.006), (0.06, 0.00005, 0.002)] and not m.phonon_single_pole
+
+
+def test_element_masses_cover_the_periodic_table():
+    assert sorted(inputs.ELEMENT_MASS) == list(range(1, 119))
+    assert inputs.ELEMENT_MASS[13] == 26.9815386 and inputs.ELEMENT_MASS[8] == 15.9992
+    m = [inputs.ELEMENT_MASS[z] for z in range(1, 119)]
+    assert all(b > a for a, b in zip(m[:17], m[1:18]))    # increasing through argon
+
+
+def test_read_dos_two_columns():
+    t = "  -0.74E+01   0.4E-02\n -7.3 0.007\n\n -7.2d0 3.7d-2\n"
+    E, D = inputs.read_dos(t)
+    assert np.array_equal(E, [-7.4, -7.3, -7.2]) and np.array_equal(D, [0.004, 0.007, 0.037])
+
+
+def test_example_material_of_the_test_cdf_workload():
+    mat, ion, opt, run = configs.config_material("test_cdf")
+    assert mat.name == "Al2O3-like oxide" and len(mat.atoms) == 2 and not mat.gap_differs
+    assert [len(a.shells) for a in mat.atoms] == [2, 1] and len(mat.phonon) == 2
+
+
+@pytest.mark.skipif(not os.path.isdir(REF_CDF), reason="the reference's input folder is not present")
+def test_reference_input_folder():
+    """Every material file the reference ships parses, or is rejected for
+    a stated reason (formula, single-pole markers, BEB shells, or values the
+    reference's reader stops on as well)."""
+    ok, rejected = 0, 0
+    for p in sorted(glob.glob(os.path.join(REF_CDF, "*.cdf"))):
+        try:
+            f = inputs.parse_cdf(open(p, errors="replace").read())
+        except ValueError as e:
+            assert any(w in str(e) for w in ("EADL", "BEB", "single-pole", "unreadable")), (p, e)
+            rejected += 1
+            continue
+        assert f.atoms and f.atoms[0]["shells"], p       # other elements may give all electrons to the VB
+        assert all(s["Ip"] >= 0.1 and s["Nel"] > 0 for a in f.atoms for s in a["shells"]), p
+        assert f.dens > 0 and math.isfinite(f.E_F), p
+        ok += 1
+    assert ok >= 10, (ok, rejected)
+    f = inputs.parse_cdf(open(os.path.join(REF_CDF, "Al2O3.cdf")).read())
+    assert [(a["Z"], a["pers"]) for a in f.atoms] == [(13, 2.0), (8, 3.0)]
+    assert [s["Ip"] for a in f.atoms for s in a["shells"]] == [1559.1, 89.0, 8.8, 538.0]
+    assert f.atoms[0]["shells"][2]["osc"][0] == (25.4, 275.0, 12.5)
+    assert f.egap == 0.1 and len(f.phonon) == 2
