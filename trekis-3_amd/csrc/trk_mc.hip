@@ -806,14 +806,25 @@ int run_mc(trk_ctx *c, int64_t nmc, uint64_t seed, int64_t first_iter, const dou
             d.out_cnt = s->d_counts + 4 * nxt;
             const int tb = mixed ? kTransportBlockMx : kTransportBlockFp64;
             int64_t nblk = (nthr + tb - 1) / tb;
+            int tbl = tb;             // threads per workgroup of this launch
+            if (nblk < s->max_blocks) {
+                // few particles (the ions' first round, the last rounds):
+                // spread them over the resident workgroups, one or a few
+                // waves each, rather than full workgroups on a few CUs (the
+                // kernel stages its tables with blockDim.x threads)
+                const int64_t waves = (nthr + 63) / 64;
+                const int64_t w = (waves + s->max_blocks - 1) / s->max_blocks;
+                tbl = (int)(64 * w);
+                nblk = (nthr + tbl - 1) / tbl;
+            }
             if (nblk > s->max_blocks) nblk = s->max_blocks;
             d.cursor = s->d_counts + 8;
             HIPC(hipMemsetAsync(d.cursor, 0, sizeof(int), st));
             if (c->prof) HIPC(hipEventRecord(c->pe[0], st));
-            if (mixed && s->all_staged) mx::k_transport<true><<<(unsigned)nblk, kTransportBlockMx, s->lds_bytes, st>>>(d);
-            else if (mixed) mx::k_transport<false><<<(unsigned)nblk, kTransportBlockMx, s->lds_bytes, st>>>(d);
-            else if (s->all_staged) fp64::k_transport<true><<<(unsigned)nblk, kTransportBlockFp64, s->lds_bytes, st>>>(d);
-            else fp64::k_transport<false><<<(unsigned)nblk, kTransportBlockFp64, s->lds_bytes, st>>>(d);
+            if (mixed && s->all_staged) mx::k_transport<true><<<(unsigned)nblk, tbl, s->lds_bytes, st>>>(d);
+            else if (mixed) mx::k_transport<false><<<(unsigned)nblk, tbl, s->lds_bytes, st>>>(d);
+            else if (s->all_staged) fp64::k_transport<true><<<(unsigned)nblk, tbl, s->lds_bytes, st>>>(d);
+            else fp64::k_transport<false><<<(unsigned)nblk, tbl, s->lds_bytes, st>>>(d);
             HIPC(hipGetLastError());
             if (c->prof) { HIPC(hipEventRecord(c->pe[1], st)); pending = 1; }
             launches++;
