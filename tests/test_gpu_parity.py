@@ -94,6 +94,22 @@ def test_mc_batching_and_replicas_exact(name):
     assert ((parts[0].n_events + parts[1].n_events) == base.n_events).all()
 
 
+def test_mc_replay_on_reference_radial_grid():
+    """The reference's own 50 radii (1 A to 10 um, Radius_for_distributions):
+    the GPU's tallies on that grid agree with the oracle's and every event
+    is the same."""
+    ctx, orc, mat, ion, opt, run, R0, V0 = setup("test")
+    R, V = tg.reference_radial_grid(mat.layer)
+    say("[mc] test on the reference radial grid")
+    g = ctx.run_mc(run["nmc"], run["seed"], R, V, trace_cap=3_000_000)
+    c = orc.run_mc(run["nmc"], run["seed"], R, V, tg.TallySet(orc.Nt, len(R), mat), trace_cap=3_000_000)
+    assert (g.n_events == c.n_events).all(), (g.n_events, c.n_events)
+    mm = first_mismatch(sort_trace(g.trace), sort_trace(c.trace))
+    assert mm is None, mm
+    bad = compare_tallies(g, c)
+    assert not bad, bad
+
+
 def test_mc_replay_partly_staged():
     """With most tables left in HBM (a small LDS budget), the transport kernel
     that tests each table for staging replays the oracle bit for bit too."""

@@ -150,3 +150,15 @@ def test_reference_input_folder():
     assert [s["Ip"] for a in f.atoms for s in a["shells"]] == [1559.1, 89.0, 8.8, 538.0]
     assert f.atoms[0]["shells"][2]["osc"][0] == (25.4, 275.0, 12.5)
     assert f.egap == 0.1 and len(f.phonon) == 2
+
+
+def test_reference_radial_grid():
+    """The 50 radii of Radius_for_distributions (SO:1395-1433) and Out_V:
+    the cylinder shells' volumes add up to the outer cylinder's."""
+    R, V = configs.reference_radial_grid(250.0)
+    exp = [1, 2, 3, 4, 5, 6, 7, 8, 9, 10, 15, 20, 30, 40, 50, 60, 70, 80, 90, 100,
+           150, 200, 300, 400, 500, 600, 700, 800, 900, 1000, 1500, 2000, 3000, 4000, 5000, 6000, 7000, 8000,
+           9000, 10000, 15000, 20000, 30000, 40000, 50000, 60000, 70000, 80000, 90000, 100000]
+    assert R.tolist() == [float(x) for x in exp]
+    assert V[0] == 1.0 / (1.0 * configs.G_PI * 250.0)
+    assert np.isclose(np.sum(1.0 / V), configs.G_PI * 1.0e10 * 250.0, rtol=1e-12)

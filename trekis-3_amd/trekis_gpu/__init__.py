@@ -183,4 +183,4 @@ class TallySet:
         return np.array(self.s.n_events[:], dtype=np.int64)
 
 
-from .configs import MaterialDesc, Shell, Atom, radial_grid, config_material  # noqa: E402,F401
+from .configs import MaterialDesc, Shell, Atom, radial_grid, reference_radial_grid, config_material  # noqa: E402,F401

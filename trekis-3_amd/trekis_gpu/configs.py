@@ -207,6 +207,32 @@ def radial_grid(NR, Rmin, Rmax, layer):
     return R, V
 
 
+def reference_radial_grid(layer):
+    """The reference's own 50 cylinder radii (Radius_for_distributions,
+    Sorting_output_data.f90:1363-1435): 1..10 A in steps of 1, then per
+    decade a first radius of 1.5 and steps of 1 times the decade (15, 20..100,
+    150, 200..1000, ..., 15000, 20000..100000 A), and Out_V as radial_grid
+    computes it."""
+    R = []
+    for i in range(1, 51):
+        if i < 11:
+            r = i
+        elif i < 41:
+            d = 10 ** ((i - 1) // 10)            # 10, 100, 1000
+            k = i - 10 * ((i - 1) // 10)
+            r = 15 * d // 10 if k == 1 else k * d
+        else:
+            r = 15000 if i == 41 else (i - 40) * 10000
+        R.append(float(r))
+    R = np.array(R)
+    V = np.zeros(len(R))
+    R0 = 0.0
+    for j in range(len(R)):
+        V[j] = 1.0 / ((R[j] * R[j] - R0 * R0) * G_PI * layer)
+        R0 = R[j]
+    return R, V
+
+
 # -------------------------------------------------------------- DOS shapes
 def dos_insulator(width=22.0, step=0.05):
     """Synthetic two-band oxide VB: an s-like band (Gaussian at -18 eV) under a
