@@ -23,6 +23,9 @@ Tolerances, per radial tally (time x radius; hole tallies summed over shells):
     track are carried by a few fast electrons, which one flipped history
     moves by more; they enter through the next bound only;
   - relative L1 difference over all bins and nodes: 5e-3.
+The workloads: the Al-like metal, the Al2O3-like target with the Au ion,
+the SiO2-like target with phonon scattering and photons, and the material
+read from the example full-CDF file (test_cdf).
 Photon tallies of these workloads hold only a few photon counts at the time
 nodes and are not checked per bin.  The total deposited energy (Out_tot_E)
 agrees to 1e-5 relative: the ion's energy transfers are sampled in FP32 and
@@ -40,7 +43,7 @@ pytestmark = pytest.mark.gpu
 RADIAL = {"Out_ne": "Out_ne", "Out_Ee": "Out_ne", "Out_Elat": "Out_ne",
           "Out_nh": "Out_nh", "Out_Eh": "Out_nh", "Out_Ehkin": "Out_nh"}
 MIN_COUNTS = 1000
-CASES = {"test_metal": 256, "config2": 512, "config5": 16}
+CASES = {"test_metal": 256, "config2": 512, "config5": 16, "test_cdf": 256}
 
 
 def _mixed_context(name):
