@@ -151,3 +151,20 @@ def test_reference_input_folder():
     assert f.atoms[0]["shells"][2]["osc"][0] == (25.4, 275.0, 12.5)
     assert f.egap == 0.1 and len(f.phonon) == 2
 
+
+
+@pytest.mark.skipif(not os.path.isdir(REF_CDF), reason="the reference's input folder is not present")
+def test_reference_material_and_dos_prepare():
+    """The reference's Al2O3 material and DOS files, read and prepared by the
+    library's host functions (DOS regridded on 0.1 eV from the VB top, the
+    integrated DOS holding the VB electrons)."""
+    import trekis_gpu as tg
+    dos = inputs.read_dos(os.path.join(os.path.dirname(REF_CDF), "INPUT_DOS", "Al2O3.dos"))
+    assert len(dos[0]) > 50 and np.all(np.diff(dos[0]) > 0)
+    mat = inputs.read_cdf(os.path.join(REF_CDF, "Al2O3.cdf"), layer=100.0, dos=dos)
+    assert mat.gap_differs and mat.egap_file == 0.1          # the density line's quirk, see parse_cdf
+    mat.prepare(tg.load_library())
+    E = mat.dos["E"]
+    assert mat.n_dos == int(np.floor((dos[0][-1] - dos[0][0]) / 0.1))
+    assert E[0] == 0.0 or E[0] < 0.1
+    assert np.all(mat.dos["int_DOS"] >= 0.0)
