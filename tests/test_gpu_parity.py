@@ -19,10 +19,10 @@ from gpu_helpers import setup, say, sort_trace, first_mismatch, compare_tallies
 
 pytestmark = pytest.mark.gpu
 
-SMALL = ["test", "test_metal", "test_sio2", "test_cdf"]
+SMALL = ["test", "test_metal", "test_sio2", "test_cdf", "test_polymer"]
 FULL = ["config1", "config2"]       # the Al-like/Xe and Al2O3-like/Au workloads at full size
 # table parity: test_sio2 has config5's tables (SiO2-like, U 20 MeV/u, phonon CDF, photons)
-TABLES = ["test", "test_metal", "test_cdf"] + FULL + ["config5"]
+TABLES = ["test", "test_metal", "test_cdf", "test_polymer"] + FULL + ["config5"]
 
 
 @pytest.mark.parametrize("name", TABLES)

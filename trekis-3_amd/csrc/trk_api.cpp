@@ -35,7 +35,9 @@ static int validate(const trk_material *m, const trk_ion *ion, const trk_options
     if (m->n_atoms < 1 || m->n_atoms > TRK_MAX_ATOMS) { why = "n_atoms out of range"; return 1; }
     int nsh = 0, nosc = 0;
     for (int a = 0; a < m->n_atoms; ++a) {
-        if (m->n_shells[a] < 1) { why = "every atom needs at least one shell"; return 1; }
+        // other atoms may give all their electrons to the valence band (the
+        // reference's hydrocarbon files list no shell for hydrogen)
+        if (m->n_shells[a] < (a == 0 ? 1 : 0)) { why = "the first atom needs at least one shell (the valence band)"; return 1; }
         nsh += m->n_shells[a];
     }
     if (nsh > TRK_MAX_SHELLS) { why = "too many shells"; return 1; }
