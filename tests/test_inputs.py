@@ -168,3 +168,59 @@ def test_reference_material_and_dos_prepare():
     assert mat.n_dos == int(np.floor((dos[0][-1] - dos[0][0]) / 0.1))
     assert E[0] == 0.0 or E[0] < 0.1
     assert np.all(mat.dos["int_DOS"] >= 0.0)
+
+
+PARAMS = """Test    ! material name
+79         ! SHI atomic number
+2166.7     ! [MeV] total SHI energy
+-1         ! [proton mass], negative: the element's
+1000.0     ! [fs] total time
+2000.0 0   ! [fs] timestep; kind of time grid
+10.0d0     ! [eV] cut-off
+100.0      ! [A] layer
+0.0        ! [K] temperature
+4   0.0    ! Zeff model; fixed value
+0          ! point-like ion
+0   0      ! elastic: Mott; target Zeff
+7   -1     ! dispersion relation; m_eff
+1          ! plasmon integration limit
+1.0        ! hole mass
+1          ! photons
+-1.0 10.0d0 6.18d0  ! work function, barrier length, height
+10000      ! iterations
+-1         ! threads
+
+!----------------
+DOS Other.dos
+!CDF Ignored.cdf
+"""
+
+
+def test_parameter_file_fields_and_corrections():
+    """Read_input_file's order and corrections (RF:256-372)."""
+    p = inputs.parse_parameters(PARAMS)
+    assert p["material"] == "Test" and p["Z"] == 79 and p["E_MeV"] == 2166.7
+    assert p["mass"] == inputs.ELEMENT_MASS[79]           # negative mass: the element's
+    assert p["dt"] == 1000.0 and p["dt_flag"] == 0         # dt at most Tim on a linear grid
+    assert p["fixed_Zeff"] == 79.0                         # fixed Zeff within (0, Z]
+    assert p["kind_of_DR"] == 1 and p["El_eff_mass"] == 1.0
+    assert p["plasmon_Emax"] == 1 and p["include_photons"] == 1 and p["work_function"] == 0.0
+    assert p["NMC"] == 10000 and p["DOS_file"] == "Other.dos" and p["CDF_file"] is None
+    q = inputs.parse_parameters(PARAMS.replace("2000.0 0   !", "1.5 1   !"))
+    assert q["dt"] == 2.0 and q["dt_flag"] == 1           # at least 2 on a logarithmic grid
+    with pytest.raises(ValueError, match="ends early"):
+        inputs.parse_parameters("\n".join(PARAMS.splitlines()[:8]))
+
+
+@pytest.mark.skipif(not os.path.isdir(REF_CDF), reason="the reference's input folder is not present")
+def test_reference_input_tree_loads():
+    """The reference's own INPUT_PARAMETERS.txt with its material and DOS
+    files: material, ion and options, prepared by the library's host code."""
+    import trekis_gpu as tg
+    root = os.path.dirname(REF_CDF)
+    mat, ion, opt, nmc = inputs.load_inputs(os.path.join(root, "INPUT_PARAMETERS.txt"))
+    p = inputs.parse_parameters(open(os.path.join(root, "INPUT_PARAMETERS.txt")).read())
+    assert mat.name and ion.Zat == p["Z"] and ion.E == p["E_MeV"] * 1.0e6 and nmc == p["NMC"]
+    assert (opt.Tim, opt.dt, opt.dt_flag) == (p["Tim"], p["dt"], p["dt_flag"])
+    mat.prepare(tg.load_library())
+    assert mat.n_dos > 10

@@ -186,7 +186,7 @@ class MaterialDesc:
         m.v_f = self.v_f
         m.cut_off = self.cut_off
         m.temp = self.temp
-        m.work_function = 0.0
+        m.work_function = getattr(self, "work_function", 0.0)   # the library refuses emission (> 0)
         m.n_dos = self.n_dos
         for k, f in [("E", "dos_E"), ("DOS", "dos_DOS"), ("int_DOS", "dos_int_DOS"), ("k", "dos_k"),
                      ("Eff_m", "dos_Eff_m"), ("k_inv", "dos_k_inv"), ("Eff_m_inv", "dos_Eff_m_inv")]:

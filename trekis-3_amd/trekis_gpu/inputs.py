@@ -233,3 +233,110 @@ def read_cdf(path_or_text: str, layer: float, dos: Tuple[np.ndarray, np.ndarray]
     mat.egap_file = f.egap
     mat.gap_differs = not math.isclose(f.egap, atoms[0].shells[-1].Ip, rel_tol=0.0, abs_tol=0.0)
     return mat
+
+
+# -------------------------------------------------------------- run parameters
+class Parameters(dict):
+    """Fields of the reference's run-parameter file, after its own checks."""
+
+
+def parse_parameters(text: str) -> Parameters:
+    """The run-parameter file (INPUT_PARAMETERS.txt), read as Read_input_file
+    reads it (Reading_files_and_parameters.f90:256-372): one list-directed
+    READ per line in a fixed order, with the reference's corrections: the
+    ion's mass from the element table unless a positive mass is given, dt
+    at most Tim on a linear grid and at least 2 on a logarithmic one, a
+    fixed Zeff within (0, Z], dispersion kind 1 and electron mass 1 where
+    the values are out of range, no surface emission unless the work
+    function, barrier length and height are all positive.  After the fixed
+    lines, 'DOS <file>' and 'CDF <file>' name other material files."""
+    lines = text.splitlines()
+    pos = 0
+
+    def read(kinds):
+        nonlocal pos
+        items = []
+        while len(items) < len(kinds):
+            if pos >= len(lines):
+                raise ValueError("unreadable parameter file: it ends early")
+            items += _items(lines[pos])
+            pos += 1
+        out = []
+        for k, t in zip(kinds, items):
+            if k == "s":
+                out.append(t)
+            else:
+                try:
+                    out.append((_fortran_int if k == "i" else _fortran_real)(t))
+                except ValueError as e:
+                    raise ValueError(f"unreadable parameter file: line {pos}: {e}") from None
+        return out
+
+    p = Parameters()
+    (p["material"],) = read("s")
+    (Z,) = read("i")
+    p["Z"] = Z
+    (p["E_MeV"],) = read("r")
+    (M,) = read("r")
+    p["mass"] = M if M > 0.0 else ELEMENT_MASS[abs(Z)]
+    (p["Tim"],) = read("r")
+    dt, flag = read("ri")
+    if dt > p["Tim"] and flag <= 0:
+        dt = p["Tim"]
+    if dt < 2 and flag >= 1:
+        dt = 2.0
+    p["dt"], p["dt_flag"] = dt, flag
+    (p["cut_off"],) = read("r")
+    (p["layer"],) = read("r")
+    (p["temp"],) = read("r")
+    kz, fz = read("ir")
+    if fz <= 0.0 or fz > Z:
+        fz = float(Z)
+    p["Kind_Zeff"], p["fixed_Zeff"] = kz, fz
+    (p["Kind_ion"],) = read("i")
+    p["kind_of_EMFP"], p["CDF_elast_Zeff"] = read("ii")
+    dr, meff = read("ir")
+    p["kind_of_DR"] = dr if 0 < dr <= 4 else 1
+    p["El_eff_mass"] = meff if meff >= 0 else 1.0
+    (pl,) = read("i")
+    p["plasmon_Emax"] = int(pl == 1)
+    (p["hole_mass"],) = read("r")
+    (ph,) = read("i")
+    p["include_photons"] = int(ph == 1)
+    wf, bl, bh = read("rrr")
+    p["work_function"] = wf if (wf > 0.0 and bl > 0.0 and bh > 0.0) else 0.0
+    (p["NMC"],) = read("i")
+    (p["Num_th"],) = read("i")
+    p["DOS_file"] = p["CDF_file"] = None
+    for ln in lines[pos:]:
+        it = _items(ln)
+        if len(it) >= 2 and not it[0].startswith("!") and it[0] in ("DOS", "CDF"):
+            p[it[0] + "_file"] = it[1]
+    return p
+
+
+def load_inputs(parameters_path: str, root: str = None):
+    """(MaterialDesc, Ion, Options, NMC) from the reference's input tree: the
+    parameter file and, under root (default: the parameter file's folder),
+    INPUT_CDF/<material>.cdf and INPUT_DOS/<material>.dos (or the files the
+    parameter file names).  Options this library does not compute (surface
+    emission, Brandt-Kitagawa ions, DSF elastic tables) are refused by the
+    library when the context is created."""
+    import os
+    from . import abi
+    root = root or os.path.dirname(os.path.abspath(parameters_path))
+    p = parse_parameters(_text(parameters_path))
+    cdf = os.path.join(root, "INPUT_CDF", p["CDF_file"] or p["material"] + ".cdf")
+    dos = os.path.join(root, "INPUT_DOS", p["DOS_file"] or p["material"] + ".dos")
+    mat = read_cdf(cdf, layer=p["layer"], dos=read_dos(dos), cut_off=p["cut_off"], hole_mass=p["hole_mass"],
+                   el_eff_mass=p["El_eff_mass"])
+    mat.temp = p["temp"]
+    mat.work_function = p["work_function"]
+    ion = abi.Ion()
+    ion.Zat, ion.Mass, ion.E = p["Z"], p["mass"], p["E_MeV"] * 1.0e6
+    ion.Kind_Zeff, ion.fixed_Zeff, ion.Kind_ion = p["Kind_Zeff"], p["fixed_Zeff"], p["Kind_ion"]
+    opt = abi.Options()
+    opt.kind_of_EMFP, opt.CDF_elast_Zeff = p["kind_of_EMFP"], p["CDF_elast_Zeff"]
+    opt.kind_of_DR, opt.plasmon_Emax, opt.include_photons = p["kind_of_DR"], p["plasmon_Emax"], p["include_photons"]
+    opt.Tim, opt.dt, opt.dt_flag = p["Tim"], p["dt"], p["dt_flag"]
+    return mat, ion, opt, p["NMC"]
