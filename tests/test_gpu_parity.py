@@ -110,6 +110,33 @@ def test_mc_replay_on_reference_radial_grid():
     assert not bad, bad
 
 
+def test_input_tree_replay_bitwise():
+    """A run set up from an input tree laid out like the reference's
+    (trekis_gpu/examples: INPUT_PARAMETERS.txt, INPUT_CDF/, INPUT_DOS/) on
+    the reference's radial grid: every event equals the oracle's and the
+    tallies agree."""
+    import os
+    from oracle import Oracle
+    from trekis_gpu import inputs
+    lib = tg.load_library()
+    path = os.path.join(os.path.dirname(os.path.abspath(tg.__file__)), "examples", "INPUT_PARAMETERS.txt")
+    mat, ion, opt, nmc = inputs.load_inputs(path)
+    mat.prepare(lib)
+    say(f"[mc] input tree: {mat.name}, Z={ion.Zat}, {nmc} iterations")
+    ctx = tg.Context(mat, ion, opt)
+    ctx.build_tables()
+    orc = Oracle(mat, ion, opt)
+    orc.inject_from(ctx)
+    R, V = tg.reference_radial_grid(mat.layer)
+    g = ctx.run_mc(nmc, 0x5EED0008, R, V, trace_cap=3_000_000)
+    c = orc.run_mc(nmc, 0x5EED0008, R, V, tg.TallySet(orc.Nt, len(R), mat), trace_cap=3_000_000)
+    assert (g.n_events == c.n_events).all(), (g.n_events, c.n_events)
+    mm = first_mismatch(sort_trace(g.trace), sort_trace(c.trace))
+    assert mm is None, mm
+    bad = compare_tallies(g, c)
+    assert not bad, bad
+
+
 def test_mc_replay_partly_staged():
     """With most tables left in HBM (a small LDS budget), the transport kernel
     that tests each table for staging replays the oracle bit for bit too."""

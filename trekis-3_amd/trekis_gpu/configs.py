@@ -332,9 +332,9 @@ def config_material(name):
     test_sio2  config5 on a 100 A track segment, 2 iterations (the full
              tables; a cascade the CPU oracle replays in seconds)
     test_polymer  a material file whose hydrogen lists no shell
-             (examples/c2h4_like.cdf), O ion at 0.5 MeV/u, 0-100 fs, 4 iterations
+             (examples/INPUT_CDF/c2h4_like.cdf), O ion at 0.5 MeV/u, 0-100 fs, 4 iterations
     test_cdf  a material read from a full-CDF material file
-             (examples/al2o3_like.cdf), O ion at 0.5 MeV/u, 0-100 fs, phonon
+             (examples/INPUT_CDF/al2o3_like.cdf), O ion at 0.5 MeV/u, 0-100 fs, phonon
              CDF elastic scattering and photons, 4 iterations
     """
     opt = abi.Options()
@@ -376,10 +376,10 @@ def config_material(name):
         run = dict(nmc=2, NR=40, Rmin=1.0, Rmax=1000.0, seed=0x5EED0005)
     elif name == "test_cdf":
         # the material read from a file in the reference's full-CDF format
-        # (examples/al2o3_like.cdf: the Al2O3-like target's shells with the
+        # (examples/INPUT_CDF/al2o3_like.cdf: the Al2O3-like target's shells with the
         # oscillator strengths written out, two phonon peaks)
         from .inputs import read_cdf
-        path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "examples", "al2o3_like.cdf")
+        path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "examples", "INPUT_CDF", "al2o3_like.cdf")
         mat = read_cdf(path, layer=100.0, dos=dos_insulator(), hole_mass=1.0, el_eff_mass=1.0)
         ion.Zat, ion.Mass, ion.E = 8, 16.00, 0.5e6 * 16.0
         opt.kind_of_EMFP, opt.CDF_elast_Zeff = 1, 1
@@ -389,10 +389,10 @@ def config_material(name):
         run = dict(nmc=4, NR=40, Rmin=1.0, Rmax=1000.0, seed=0x5EED0006)
     elif name == "test_polymer":
         # a material whose second element lists no shell (examples/
-        # c2h4_like.cdf: hydrogen's electrons are in the valence band), Mott
+        # INPUT_CDF/c2h4_like.cdf: hydrogen's electrons are in the valence band), Mott
         # elastic scattering, no phonon block (single-pole phonon CDF)
         from .inputs import read_cdf
-        path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "examples", "c2h4_like.cdf")
+        path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "examples", "INPUT_CDF", "c2h4_like.cdf")
         mat = read_cdf(path, layer=100.0, dos=dos_insulator(), hole_mass=1.0, el_eff_mass=1.0)
         ion.Zat, ion.Mass, ion.E = 8, 16.00, 0.5e6 * 16.0
         opt.kind_of_EMFP, opt.CDF_elast_Zeff = 0, 0
