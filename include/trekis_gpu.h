@@ -55,7 +55,7 @@ typedef struct trk_material {
     int32_t Zat[TRK_MAX_ATOMS];
     double  Mass[TRK_MAX_ATOMS];        /* [proton masses] */
     double  Pers[TRK_MAX_ATOMS];        /* atoms of this kind per formula unit */
-    int32_t n_shells[TRK_MAX_ATOMS];
+    int32_t n_shells[TRK_MAX_ATOMS];    /* >= 1 for atom 0 (its last shell is the VB); 0 allowed for the others */
 
     double  Ip[TRK_MAX_SHELLS];         /* [eV] ionisation potential (>= 0.1, RF:1554) */
     double  Nel[TRK_MAX_SHELLS];        /* electrons in the shell */
