@@ -236,6 +236,16 @@ double trk_rng_uniform(uint64_t seed, uint64_t particle_id, uint32_t iteration, 
  * 5 alpha, 6 c, 7 a0), restating Universal_Constants.f90:28-60. */
 double trk_constant(int which);
 
+/* Decay-time rules of check_atomic_parameters (Dealing_with_EADL.f90:343-372)
+ * applied in place to a caller's descriptor: Radiat = 1e23 fs for the valence
+ * band (the last shell of atom 0, or Shl_num >= 63) and for every shell when
+ * include_photons is 0; Auger = 1e23 fs for the valence band.  A deep shell
+ * whose Auger time is <= 0 or > 1e30 fs (the reference's "read it from EADL"
+ * marks), or any other time that is not positive and finite, is refused
+ * (TRK_ERR_ARG, reason in why).  trk_ctx_create applies the same rules to its
+ * copy of the descriptor. */
+int  trk_check_atomic_parameters(trk_material *m, int32_t include_photons, char *why, size_t n);
+
 /* f-sum and k-sum rules of a set of Drude-Lorentz oscillators
  * (sumrules, Cross_sections.f90:728-749). */
 int  trk_sumrules(const double *A, const double *E0, const double *Gamma, int32_t n,

@@ -140,6 +140,7 @@ static double rn(or_mc *S, uint64_t id, uint32_t *nd)
 /* Next_free_path_2d, MC:1874-1899 */
 double or_nfp_2d(or_ctx *c, double E, const double *Earr, const double *Larr, int32_t n)
 {
+    if (n < 1) return 1.0e30;   /* no table (no photons): an infinite path */
     int N = or_find_mono_2d(Earr, n, E);
     if (N < 0) { c->err_numeric = 1; return 1.0e30; }
     if (N == 1) {

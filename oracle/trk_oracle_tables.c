@@ -1058,6 +1058,13 @@ or_ctx *or_create(const trk_material *mat, const trk_ion *ion, const trk_options
     c->nsh = s;
     c->vb = mat->n_shells[0] - 1;
     c->Egap = mat->Ip[c->vb];
+    /* check_atomic_parameters, EADL:350-351, 366-367: no radiative decay of
+     * the valence band or without photons, no Auger decay of the valence band */
+    for (int k = 0; k < s; ++k) {
+        int vbk = (k == c->vb) || (mat->Shl_num[k] >= 63);
+        if (vbk || !opt->include_photons) c->m.Radiat[k] = 1.0e23;
+        if (vbk) c->m.Auger[k] = 1.0e23;
+    }
     double sm = 0.0, sp = 0.0, sz = 0.0;
     for (int a = 0; a < mat->n_atoms; ++a) {
         sm = sm + mat->Mass[a] * mat->Pers[a];

@@ -69,3 +69,50 @@ def test_constants_defining_expressions(lib):
     assert lib.trk_constant(4) == h
     assert lib.trk_constant(5) == e * e / (h * c * 4.0 * pi * e0)
     assert abs(1.0 / lib.trk_constant(5) - 137.036) < 1e-3
+
+
+def _decay_material(lib):
+    import trekis_gpu as tg
+    mat, ion, opt, run = tg.config_material("test")
+    mat.prepare(lib)
+    m = abi.Material.from_buffer_copy(mat.to_struct())
+    return m, ion, opt
+
+
+def test_decay_time_rules_of_check_atomic_parameters(lib):
+    """Dealing_with_EADL.f90:350-351, 366-370: no radiative decay of the valence
+    band or with photons off, no Auger decay of the valence band; an Auger time
+    the reference would read from EADL is refused."""
+    m, ion, opt = _decay_material(lib)
+    nsh = sum(m.n_shells[:m.n_atoms])
+    vb = m.n_shells[0] - 1
+    m.Radiat[0] = 5.0                # a deep shell with a finite radiative time
+    m.Auger[vb] = 3.0
+    m.Radiat[vb] = 3.0
+    why = ctypes.create_string_buffer(256)
+    # photons off: every radiative time becomes 1e23 fs
+    a = abi.Material.from_buffer_copy(m)
+    assert lib.trk_check_atomic_parameters(ctypes.byref(a), 0, why, 256) == 0
+    assert all(a.Radiat[s] == 1.0e23 for s in range(nsh))
+    assert a.Auger[vb] == 1.0e23 and a.Auger[0] == m.Auger[0]
+    # photons on: the deep shell keeps its time, the valence band does not decay
+    b = abi.Material.from_buffer_copy(m)
+    assert lib.trk_check_atomic_parameters(ctypes.byref(b), 1, why, 256) == 0
+    assert b.Radiat[0] == 5.0 and b.Radiat[vb] == 1.0e23 and b.Auger[vb] == 1.0e23
+    # 'read it from EADL' Auger marks and non-positive radiative times are refused
+    for field, v, photons in (("Auger", -3.6, 1), ("Auger", 2.0e30, 0), ("Auger", float("nan"), 0),
+                              ("Radiat", 0.0, 1), ("Radiat", float("inf"), 1)):
+        c = abi.Material.from_buffer_copy(m)
+        getattr(c, field)[0] = v
+        assert lib.trk_check_atomic_parameters(ctypes.byref(c), photons, why, 256) == abi.TRK_ERR_ARG, (field, v)
+        assert why.value, (field, v)
+    # trk_ctx_create refuses such a descriptor before it touches the device
+    c = abi.Material.from_buffer_copy(m)
+    c.Auger[0] = -1.0
+    h = ctypes.c_void_p()
+    rc = lib.trk_ctx_create(ctypes.byref(c), ctypes.byref(ion), ctypes.byref(opt), ctypes.byref(h))
+    assert rc == abi.TRK_ERR_ARG
+    buf = ctypes.create_string_buffer(512)
+    lib.trk_last_error(h, buf, 512)
+    assert b"EADL" in buf.value
+    lib.trk_ctx_destroy(h)

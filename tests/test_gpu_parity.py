@@ -21,8 +21,15 @@ pytestmark = pytest.mark.gpu
 
 SMALL = ["test", "test_metal", "test_sio2", "test_cdf", "test_polymer"]
 FULL = ["config1", "config2"]       # the Al-like/Xe and Al2O3-like/Au workloads at full size
-# table parity: test_sio2 has config5's tables (SiO2-like, U 20 MeV/u, phonon CDF, photons)
-TABLES = ["test", "test_metal", "test_cdf", "test_polymer"] + FULL + ["config5"]
+# the option branches the workloads above leave at their defaults: dispersion
+# kinds 2-4, ion charge models 1-4, a 300 K target, the plasmon integration
+# limit and logarithmic time grids (configs.OPTION_WORKLOADS)
+OPTIONS = ["opt_plasmon_pole", "opt_ritchie_warm", "opt_sg_warm", "opt_fixed_zeff"]
+# table parity: test_sio2 has config5's tables (SiO2-like, U 20 MeV/u, phonon CDF, photons).
+# opt_ritchie_warm's full oracle tables take minutes (a deterministic pow per
+# q step); its rows are compared at sampled energies below instead.
+TABLES = ["test", "test_metal", "test_cdf", "test_polymer"] + FULL + ["config5"] + \
+    ["opt_plasmon_pole", "opt_sg_warm", "opt_fixed_zeff"]
 
 
 @pytest.mark.parametrize("name", TABLES)
@@ -56,10 +63,42 @@ def test_tables_bitwise(name):
     assert nrows > 100
 
 
-@pytest.mark.parametrize("name", SMALL + FULL)
+def test_tables_sampled_rows_ritchie_dispersion():
+    """Ritchie's extended dispersion (kind_of_DR 3, CS:382-385) at 300 K: the
+    mean free path, stopping power and cumulative row of every shell at six
+    energies spread over the grid, from the oracle's single-point quadrature,
+    equal the GPU's table bit for bit."""
+    import ctypes
+    from oracle import dp
+    ctx, orc, mat, ion, opt, run, R, V = setup("opt_ritchie_warm")
+    nsh = sum(mat.n_shells)
+    ncmp = 0
+    for kind in (abi.TAB_E_INEL, abi.TAB_H_INEL):
+        Eg, Lg, Dg = ctx.table(kind, 0)
+        n = len(Eg)
+        for i in sorted({40, n // 4, n // 2, (3 * n) // 4, n - 40, n - 1}):
+            for s in range(nsh):
+                try:
+                    hg, lg = ctx.row(kind, s, i)
+                except tg.TrekisError:
+                    continue
+                say(f"[tables] opt_ritchie_warm kind {abi.TABLE_NAMES[kind]} shell {s} node {i}")
+                _, Ls, Ds = ctx.table(kind, s)
+                L, D, cnt = ctypes.c_double(), ctypes.c_double(), ctypes.c_int32()
+                hw, lam = np.zeros(len(hg) + 64), np.zeros(len(hg) + 64)
+                rc = orc.lib.or_inel_point(orc.h, kind, s, Eg[i], ctypes.byref(L), ctypes.byref(D), dp(hw), dp(lam),
+                                           len(hw), ctypes.byref(cnt))
+                assert rc == 0 and cnt.value == len(hg), (kind, s, i, rc, cnt.value, len(hg))
+                assert L.value == Ls[i] and D.value == Ds[i], (kind, s, i, L.value, Ls[i], D.value, Ds[i])
+                assert np.array_equal(hw[:len(hg)], hg) and np.array_equal(lam[:len(hg)], lg), (kind, s, i)
+                ncmp += 1
+    assert ncmp >= 12
+
+
+@pytest.mark.parametrize("name", SMALL + FULL + OPTIONS)
 def test_mc_replay_bitwise(name):
     ctx, orc, mat, ion, opt, run, R, V = setup(name)
-    nmc = run["nmc"] if name in SMALL else 1
+    nmc = 1 if name in FULL else run["nmc"]
     say(f"[mc] {name}: GPU nmc={nmc}")
     g = ctx.run_mc(nmc, run["seed"], R, V, trace_cap=3_000_000)
     say(f"[mc] {name}: oracle")
@@ -120,7 +159,9 @@ def test_input_tree_replay_bitwise():
     from trekis_gpu import inputs
     lib = tg.load_library()
     path = os.path.join(os.path.dirname(os.path.abspath(tg.__file__)), "examples", "INPUT_PARAMETERS.txt")
-    mat, ion, opt, nmc = inputs.load_inputs(path)
+    # radiative times per (element, shell), as the reference would take them
+    # from EADL: the Al L shell and O K shell decay mostly radiatively here
+    mat, ion, opt, nmc = inputs.load_inputs(path, radiat=[[5.0, 1.0e23], [2.0]])
     mat.prepare(lib)
     say(f"[mc] input tree: {mat.name}, Z={ion.Zat}, {nmc} iterations")
     ctx = tg.Context(mat, ion, opt)
@@ -131,6 +172,7 @@ def test_input_tree_replay_bitwise():
     g = ctx.run_mc(nmc, 0x5EED0008, R, V, trace_cap=3_000_000)
     c = orc.run_mc(nmc, 0x5EED0008, R, V, tg.TallySet(orc.Nt, len(R), mat), trace_cap=3_000_000)
     assert (g.n_events == c.n_events).all(), (g.n_events, c.n_events)
+    assert g.n_events[abi.EVENT_NAMES.index("radiative")] > 0
     mm = first_mismatch(sort_trace(g.trace), sort_trace(c.trace))
     assert mm is None, mm
     bad = compare_tallies(g, c)

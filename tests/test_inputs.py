@@ -13,6 +13,7 @@ import pytest
 from trekis_gpu import configs, inputs
 
 REF_CDF = "/root/reference/INPUT_CDF"
+EXAMPLE_CDF = os.path.join(os.path.dirname(os.path.abspath(configs.__file__)), "examples", "INPUT_CDF", "al2o3_like.cdf")
 
 TWO_ELEMENTS = """Test oxide   ! name
 2		! elements
@@ -142,6 +143,9 @@ def test_reference_input_folder():
             continue
         assert f.atoms and f.atoms[0]["shells"], p       # other elements may give all electrons to the VB
         assert all(s["Ip"] >= 0.1 and s["Nel"] > 0 for a in f.atoms for s in a["shells"]), p
+        # Auger times the library can use: positive, at most 1e30 fs, 1e23 fs for the band
+        assert all(0.0 < s["auger"] <= 1.0e30 for a in f.atoms for s in a["shells"]), p
+        assert f.atoms[0]["shells"][-1]["auger"] == 1.0e23, p
         assert f.dens > 0 and math.isfinite(f.E_F), p
         ok += 1
     assert ok >= 10, (ok, rejected)
@@ -218,9 +222,26 @@ def test_reference_input_tree_loads():
     files: material, ion and options, prepared by the library's host code."""
     import trekis_gpu as tg
     root = os.path.dirname(REF_CDF)
-    mat, ion, opt, nmc = inputs.load_inputs(os.path.join(root, "INPUT_PARAMETERS.txt"))
     p = inputs.parse_parameters(open(os.path.join(root, "INPUT_PARAMETERS.txt")).read())
+    assert p["include_photons"] == 1
+    # radiative decay is on there, and its times come from EADL in the reference
+    with pytest.raises(ValueError, match="radiative decay times"):
+        inputs.load_inputs(os.path.join(root, "INPUT_PARAMETERS.txt"))
+    mat, ion, opt, nmc = inputs.load_inputs(os.path.join(root, "INPUT_PARAMETERS.txt"), radiat="off")
+    assert all(s.radiat == 1.0e23 for a in mat.atoms for s in a.shells)
     assert mat.name and ion.Zat == p["Z"] and ion.E == p["E_MeV"] * 1.0e6 and nmc == p["NMC"]
     assert (opt.Tim, opt.dt, opt.dt_flag) == (p["Tim"], p["dt"], p["dt_flag"])
     mat.prepare(tg.load_library())
     assert mat.n_dos > 10
+
+
+def test_auger_times_the_reference_reads_from_eadl_are_refused():
+    """An Auger time <= 0 or > 1e30 fs in a material file makes the reference
+    read it from EADL (Dealing_with_EADL.f90:368-370); this reader refuses it.
+    The valence band never decays by Auger (1e23 fs, EADL:366-367)."""
+    text = open(EXAMPLE_CDF).read()
+    for bad in ("-3.6", "0.0", "2.0d31"):
+        with pytest.raises(ValueError, match="EADL"):
+            inputs.parse_cdf(text.replace("538.0e0\t2\t4.613e0", "538.0e0\t2\t" + bad))
+    f = inputs.parse_cdf(text.replace("24\t1.0d23", "24\t-1.0"))
+    assert f.atoms[0]["shells"][-1]["auger"] == 1.0e23

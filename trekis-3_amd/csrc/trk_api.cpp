@@ -58,7 +58,8 @@ static int validate(const trk_material *m, const trk_ion *ion, const trk_options
     if (ion->Kind_ion != 0) { why = "Brandt-Kitagawa ions are not built"; return 1; }
     if (!(o->Tim > 0.0) || !(o->dt > 0.0)) { why = "Tim and dt must be positive"; return 1; }
     if (o->dt_flag >= 1 && o->dt < 2.0) { why = "a logarithmic time grid needs dt >= 2 (RF:299)"; return 1; }
-    if (o->kind_of_DR < 1 || o->kind_of_DR > 3) { why = "kind_of_DR must be 1, 2 or 3"; return 1; }
+    // 4 and up fall to the free-electron relation, the reference's case default (CS:388-390)
+    if (o->kind_of_DR < 1 || o->kind_of_DR > 4) { why = "kind_of_DR must be 1 to 4"; return 1; }
     if (o->precision != 0 && o->precision != 1) { why = "precision must be 0 (FP64) or 1 (mixed)"; return 1; }
     return 0;
 }
@@ -77,6 +78,14 @@ int trk_ctx_create(const trk_material *mat, const trk_ion *ion, const trk_option
     if (validate(mat, ion, opt, why)) return set_error(c, TRK_ERR_ARG, "%s", why.c_str());
     HostMat &h = c->hm;
     h.m = *mat;
+    {
+        // the reference's decay-time rules on the library's copy: with them
+        // no radiative decay is drawn when photons are off (there is then no
+        // photon table to take the photon's free path from)
+        char buf[256];
+        if (trk_check_atomic_parameters(&h.m, opt->include_photons, buf, sizeof buf))
+            return set_error(c, TRK_ERR_ARG, "%s", buf);
+    }
     h.ion = *ion;
     h.o = *opt;
     const int nd = mat->n_dos;
@@ -129,8 +138,8 @@ int trk_ctx_create(const trk_material *mat, const trk_ion *ion, const trk_option
     rc |= upload(c, &b.phG, mat->ph_Gamma, (size_t)mat->n_phonon);
     rc |= upload(c, &b.Ip, mat->Ip, (size_t)s);
     rc |= upload(c, &b.Nel, mat->Nel, (size_t)s);
-    rc |= upload(c, &b.Auger, mat->Auger, (size_t)s);
-    rc |= upload(c, &b.Radiat, mat->Radiat, (size_t)s);
+    rc |= upload(c, &b.Auger, h.m.Auger, (size_t)s);
+    rc |= upload(c, &b.Radiat, h.m.Radiat, (size_t)s);
     rc |= upload(c, &b.Mass, mat->Mass, (size_t)mat->n_atoms);
     rc |= upload(c, &b.Pers, mat->Pers, (size_t)mat->n_atoms);
     {

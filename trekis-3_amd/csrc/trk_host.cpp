@@ -222,6 +222,51 @@ double trk_rng_uniform(uint64_t seed, uint64_t particle_id, uint32_t iteration, 
     return trk_uniform(seed, particle_id, iteration, draw);
 }
 
+// check_atomic_parameters, Dealing_with_EADL.f90:343-372, for decay times a
+// caller supplies: the valence band (Shl_num >= 63, or the last shell of the
+// first atom, where this library keeps it) gets Radiat = Auger = 1e23 fs, and
+// every shell gets Radiat = 1e23 fs when photons are off.  The reference reads
+// an Auger time <= 0 or > 1e30 fs, and every other radiative time, from the
+// EADL database, which this library does not have: such a shell is refused,
+// as is any time that is not a positive finite number.
+int trk_check_atomic_parameters(trk_material *m, int32_t include_photons, char *why, size_t n)
+{
+    auto fail = [&](const char *fmt, int s, double v) {
+        if (why && n) snprintf(why, n, fmt, s, v);
+        return (int)TRK_ERR_ARG;
+    };
+    if (!m || m->n_atoms < 1 || m->n_atoms > TRK_MAX_ATOMS) {
+        if (why && n) snprintf(why, n, "n_atoms out of range");
+        return TRK_ERR_ARG;
+    }
+    int nsh = 0;
+    for (int a = 0; a < m->n_atoms; ++a) {
+        if (m->n_shells[a] < 0 || nsh + m->n_shells[a] > TRK_MAX_SHELLS) {
+            if (why && n) snprintf(why, n, "shell counts out of range");
+            return TRK_ERR_ARG;
+        }
+        nsh += m->n_shells[a];
+    }
+    const int vb = m->n_shells[0] - 1;
+    for (int s = 0; s < nsh; ++s) {
+        const bool is_vb = (s == vb) || (m->Shl_num[s] >= 63);
+        if (is_vb || !include_photons) {
+            m->Radiat[s] = 1.0e23;                               // EADL:350-351
+        } else if (!(m->Radiat[s] > 0.0) || !std::isfinite(m->Radiat[s])) {
+            return fail("shell %d: radiative decay time %g fs must be positive and finite "
+                        "(the reference reads it from the EADL database)", s, m->Radiat[s]);
+        }
+        if (is_vb) {
+            m->Auger[s] = 1.0e23;                                // EADL:366-367
+        } else if (!(m->Auger[s] > 0.0) || m->Auger[s] > 1.0e30 || !std::isfinite(m->Auger[s])) {
+            return fail("shell %d: Auger decay time %g fs is outside (0, 1e30]; the reference then "
+                        "reads it from the EADL database, which this library does not have", s, m->Auger[s]);
+        }
+    }
+    if (why && n) why[0] = 0;
+    return TRK_OK;
+}
+
 // Physical constants as the library folds them (for checking against other
 // restatements of Universal_Constants.f90).
 double trk_constant(int which)

@@ -33,148 +33,10 @@
 #include <cstring>
 #include <string>
 #include <vector>
-#include "trk_ctx.h"
+#include "trk_mc.h"
 
 namespace trk {
 
-// ---------------------------------------------------------------- layouts
-struct PList {            // SoA particle list
-    double *E, *t0, *tn, *X, *Y, *Z, *L, *th, *ph, *Ehkin, *Mass;
-    int *sh;
-    uint64_t *id;
-    uint32_t *nd, *ne, *iter;
-};
-
-struct Tally {            // fixed-point difference arrays, replicated R times
-    int Nt, NR, nsh, R;
-    unsigned long long *cnt_e, *E_e;               // [R][Nt][NR], [R][Nt][NR][2]
-    unsigned long long *cnt_h, *E_h, *Ek_h;        // [R][Nt][NR][nsh] (x2 for energies)
-    unsigned long long *cnt_p, *E_p;               // [R][Nt][NR]
-    unsigned long long *Elat;                      // [R][Nt][NR][2] (not differenced)
-};
-
-// Per-iteration snapshot histograms for the tallies the reference normalises
-// inside each iteration (Out_Ee_vs_E, Out_Eh_vs_E, Out_theta, Out_theta_h by
-// that iteration's particle counts, Out_diff_coeff by its VB-hole count),
-// difference arrays over time like the Tally arrays.  b = iteration - iter0.
-struct Spec {
-    int on, B, ND, iter0;
-    int *nel, *nvb, *dcn;          // [B][Nt]
-    int *eve;                      // [B][Nt][NR]   electron spectrum, bins of Out_R
-    int *ehve;                     // [B][Nt][ND]   VB-hole spectrum, bins of the DOS grid
-    int *th, *thh;                 // [B][Nt][180]  polar angle of electrons / moving holes
-    unsigned long long *dcs;       // [B][Nt][2]    sum of v*L/3 of mobile holes, fixed point
-};
-
-// tables staged in each transport workgroup's LDS (see stage_tables)
-constexpr int kStaged = 26;
-
-struct McDev {
-    MatDev m;
-    // tables
-    int nE; const double *E, *Lsh, *el_imfp; const int64_t *roff; const double *rhw, *rlam;
-    int nEel; const double *Eel, *Lel; const int64_t *roff_el; const double *rhw_el, *rlam_el;
-    int nH; const double *EH, *LHsh, *hole_imfp; const int64_t *roff_h; const double *rhw_h, *rlam_h;
-    int nHel; const double *EHel, *LHel; const int64_t *roff_hel; const double *rhw_hel, *rlam_hel;
-    // 1 per cumulative row whose lam never increases (counted search)
-    const unsigned char *rmono, *rmono_h, *rmono_el, *rmono_hel;
-    int nP; const double *EP, *LPsh, *phot_imfp;
-    int nS; const double *ES, *LSsh, *shi_path;
-    const int *sd_n; const int64_t *sd_off; const double *sd_E, *sd_L, *sd_inv; const int *sd_nonmono;
-    // logs of grids and fused tables (interpolation without per-node logs)
-    const double *lnE, *ln_el_imfp, *lnEel, *ln_Lel, *lnEH, *ln_hole_imfp, *lnEHel, *ln_LHel;
-    // logs of the per-shell paths and of the cumulative transfer rows
-    const double *ln_Lsh, *ln_LHsh;
-    const double *ln_rhw, *ln_rlam, *ln_rhw_h, *ln_rlam_h, *ln_rhw_el, *ln_rlam_el, *ln_rhw_hel, *ln_rlam_hel;
-    // run
-    uint64_t seed;
-    double T_last, Tim, cutoff_eff;
-    int Nt; const double *tim_glob;
-    double inv_dt;                      // 1/dt of a linear time grid, 0 otherwise
-    int hints;                          // 1: the electron and hole energy grids, the time grid and Out_R are strictly increasing
-    // tables copied into each transport workgroup's LDS (count and offset in
-    // doubles; count 0 = read from global memory), see kStage* below
-    int lds_n[kStaged], lds_off[kStaged];
-    int NR; const double *R;
-    int include_photons;
-    // lists, by kind: electrons, holes, photons, ions
-    PList in[4], out[4];
-    int n_in[4];                        // particles of the current round
-    int cap[4];                         // capacity of each list
-    int *out_cnt;                       // [4] allocation counters of the next round
-    int *cursor;                        // next item of the current round
-    int *overflow;
-    unsigned long long *counters;       // [16] event counters
-    double *trace; unsigned long long *trace_n; long long trace_cap;
-    Tally T;
-    Spec S;
-    // used by the mixed path only, and last, so that the offsets of the
-    // fields above stay those the FP64 kernel was tuned with: per-shell
-    // constants of shi_transfer (k_shi_consts), logs of the SHI grid, of
-    // its per-shell paths and of the fused path
-    const int *sd_M; const double *sd_c;
-    const double *lnES, *ln_LSsh, *ln_shi_path;
-};
-
-enum { EV_SHI = 0, EV_E_INEL, EV_E_EL, EV_H_INEL, EV_H_EL, EV_AUGER, EV_RAD, EV_PHOT };
-
-// d.overflow bits: list capacities, the per-particle event limit, NaN state.
-constexpr int kFlagPairs = 1, kFlagPhotons = 2, kFlagEvents = 4, kFlagNaN = 8;
-// Events one particle may process in one launch.  A history needs a few
-// thousand at most; the limit only guarantees that every wave ends.
-constexpr uint32_t kMaxEvents = 1u << 22;
-
-enum { K_EL = 0, K_HO = 1, K_PH = 2, K_ION = 3 };
-
-enum { C_E_EL = 0, C_E_INEL, C_H_EL, C_H_INEL, C_DEEP, C_PHOT, C_ION, C_COUNT };
-
-// A wave runs a category of events only when at least this many of its lanes
-// want it (or when it is the most wanted one); the other lanes wait with
-// their state untouched.  Rare, long branches (ionisation, decays) then run
-// for many lanes at once instead of stalling the wave at almost every step.
-constexpr int kBatchLanes = 16;
-constexpr int kMaxWait = 16;
-
-// Events one particle may take in one round.  A particle with more left is
-// written to the next round's list; this bounds the tail of every launch by
-// the cost of kRoundEvents events instead of the longest history.
-constexpr uint32_t kRoundEvents = 512;
-
-extern __shared__ double lds_tab[];
-
-// Transport workgroup of each path: one per CU (the staged tables fill most
-// of the LDS), 12 waves, 3 per SIMD (168 VGPRs).  At 16 waves the mixed
-// path would have to spill to scratch, which costs more than the fourth
-// wave gains.
-constexpr int kTransportBlockFp64 = 768;
-constexpr int kTransportBlockMx = 768;
-
-// Next_free_path_2d (MC:1874-1899) and _1d (MC:1835-1871)
-TRK_HD double nfp_2d(const double *Ea, const double *La, int n, double E)
-{
-    int N = find_mono_2d(Ea, n, E);
-    if (N < 0) return 1.0e30;
-    if (N == 1) {
-        double M = interpolate(1, Ea[0], Ea[1], La[0], La[1], E);
-        return (M < La[0]) ? La[0] : M;
-    }
-    int Nl = N - 1;
-    if (La[Nl - 1] >= 1.0e16) return La[Nl - 1];
-    return interpolate(5, Ea[Nl - 1], Ea[N - 1], La[Nl - 1], La[N - 1], E);
-}
-
-// The event code, twice: deterministic FP64 (fp64::) and mixed precision (mx::).
-namespace fp64 {
-#define TRK_MX 0
-#include "trk_events.inc"
-#undef TRK_MX
-}  // namespace fp64
-
-namespace mx {
-#define TRK_MX 1
-#include "trk_events.inc"
-#undef TRK_MX
-}  // namespace mx
 
 // ---------------------------------------------------------------- per-iteration tallies
 // Prefix sums over time of [B][Nt][ncol] difference arrays, one thread per column.

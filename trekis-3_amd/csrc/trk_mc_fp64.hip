@@ -1,0 +1,13 @@
+// trk_mc_fp64.hip -- the event code of trk_events.inc compiled as the deterministic FP64 path (replays the oracle bit for bit).
+#include "trk_mc.h"
+
+namespace trk {
+namespace fp64 {
+#define TRK_MX 0
+#include "trk_events.inc"
+#undef TRK_MX
+
+template __global__ void k_transport<true>(McDev din);
+template __global__ void k_transport<false>(McDev din);
+}  // namespace fp64
+}  // namespace trk

@@ -1,0 +1,13 @@
+// trk_mc_mx.hip -- the event code of trk_events.inc compiled as the mixed-precision path (FP32 sampling, FP64 state, ledger and tallies).
+#include "trk_mc.h"
+
+namespace trk {
+namespace mx {
+#define TRK_MX 1
+#include "trk_events.inc"
+#undef TRK_MX
+
+template __global__ void k_transport<true>(McDev din);
+template __global__ void k_transport<false>(McDev din);
+}  // namespace mx
+}  // namespace trk

@@ -18,6 +18,7 @@ K_SHI, K_TRANSPORT = 0, 1
 TABLE_NAMES = ["e_inel", "h_inel", "e_el", "h_el", "phot", "shi", "shi_diff"]
 EVENT_NAMES = ["shi", "e_inel", "e_el", "h_inel", "h_el", "auger", "radiative", "photon"]
 
+TRK_OK, TRK_ERR_ARG, TRK_ERR_DEVICE, TRK_ERR_STATE, TRK_ERR_NUMERIC, TRK_ERR_CAPACITY = range(6)
 STATUS = {0: "ok", 1: "invalid argument", 2: "device error", 3: "call out of order",
           4: "NaN in a table search", 5: "capacity exceeded"}
 
@@ -148,6 +149,7 @@ def declare_library(lib):
         "trk_det_pow": (c_double, [c_double, c_double]),
         "trk_rng_uniform": (c_double, [c_uint64, c_uint64, c_uint32, c_uint32]),
         "trk_constant": (c_double, [c_int]),
+        "trk_check_atomic_parameters": (c_int, [P(Material), c_int32, c_char_p, c_size_t]),
         "trk_sumrules": (c_int, [DP, DP, DP, c_int32, c_double, c_double, DP, DP]),
         "trk_w_plasma": (c_double, [c_double, c_double]),
         "trk_dos_size": (c_int, [DP, c_int32, P(c_int32)]),

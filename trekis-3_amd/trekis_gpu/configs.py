@@ -399,8 +399,40 @@ def config_material(name):
         opt.Tim, opt.dt, opt.dt_flag = 100.0, 1.0, 0
         opt.e_grid_Emax = 2000.0
         run = dict(nmc=4, NR=40, Rmin=1.0, Rmax=1000.0, seed=0x5EED0007)
+    elif name in OPTION_WORKLOADS:
+        # the small workloads with the reference's option branches that the
+        # configurations above leave at their defaults switched on
+        base, dr, kz, fz, temp, plasmon, dt_flag, dt = OPTION_WORKLOADS[name]
+        mat, ion, opt, run = config_material(base)
+        mat.temp = temp
+        ion.Kind_Zeff, ion.fixed_Zeff = kz, fz
+        opt.kind_of_DR, opt.plasmon_Emax = dr, plasmon
+        opt.dt_flag, opt.dt = dt_flag, dt
+        return mat, ion, opt, run
     else:
         raise KeyError(name)
     ion.Kind_Zeff, ion.fixed_Zeff, ion.Kind_ion = 0, float(ion.Zat), 0
     opt.kind_of_DR = 1
     return mat, ion, opt, run
+
+
+# Option branches (Flag / SHI fields, Objects.f90:44-53, 135-180) on the two
+# small workloads: (base workload, kind_of_DR, Kind_Zeff, fixed Zeff, target
+# temperature [K], plasmon_Emax, dt_flag, dt).
+#   opt_plasmon_pole  plasmon-pole dispersion (CS:379-381), Bohr ion charge,
+#                     plasmon integration limit (CS:930-936), logarithmic time
+#                     grid of ratio 2 (MC:2132-2144)
+#   opt_ritchie_warm  Ritchie's extended dispersion (CS:382-385),
+#                     Nikolaev-Dmitriev charge, a 300 K target (Bose factor
+#                     of the inelastic cross sections, CS:2274-2278)
+#   opt_sg_warm       Schiwietz-Grande charge, 300 K, the reference's case
+#                     default dispersion (kind 4: free electron)
+#   opt_fixed_zeff    a fixed ion charge of 3.5, a 300 K metal (Bose factor of
+#                     the phonon CDF, CS:3290-3295), logarithmic time grid of
+#                     ratio 10 (the reference's own INPUT_PARAMETERS.txt:6 setting)
+OPTION_WORKLOADS = {
+    "opt_plasmon_pole": ("test", 2, 1, 8.0, 0.0, 1, 1, 2.0),
+    "opt_ritchie_warm": ("test", 3, 2, 8.0, 300.0, 0, 0, 1.0),
+    "opt_sg_warm": ("test", 4, 3, 8.0, 300.0, 0, 0, 1.0),
+    "opt_fixed_zeff": ("test_metal", 1, 4, 3.5, 300.0, 0, 1, 10.0),
+}

@@ -174,6 +174,18 @@ def _parse_cdf(text: str) -> CDFFile:
             osc = [tuple(read("rrr")) for _ in range(ncdf)]     # E0, A, Gamma
             at["shells"].append(dict(ncdf=ncdf, shl_num=abs(shl_num), Ip=max(Ip, 0.1), Nel=Nel, auger=auger,
                                      osc=osc))
+    # Auger times as check_atomic_parameters takes them (Dealing_with_EADL.f90
+    # :366-370): none for the valence band (designator >= 63, or the last shell
+    # of the first element, where this library keeps the band); a time <= 0 or
+    # > 1e30 fs tells the reference to read it from EADL
+    for j, at in enumerate(f.atoms):
+        for k, s in enumerate(at["shells"]):
+            vb = s["shl_num"] >= 63 or (j == 0 and k == len(at["shells"]) - 1)
+            if vb:
+                s["auger"] = 1.0e23
+            elif not (0.0 < s["auger"] <= 1.0e30):
+                raise ValueError(f"element {at['Z']} shell {k + 1}: Auger time {s['auger']} fs means 'take it from "
+                                 "the EADL database', which this library does not read; give the time in fs")
     # phonon peaks: a count and as many oscillators, or the end of the file
     # (then the single-pole phonon CDF, RF:1584-1621)
     try:
@@ -315,21 +327,29 @@ def parse_parameters(text: str) -> Parameters:
     return p
 
 
-def load_inputs(parameters_path: str, root: str = None):
+def load_inputs(parameters_path: str, root: str = None, radiat=None):
     """(MaterialDesc, Ion, Options, NMC) from the reference's input tree: the
     parameter file and, under root (default: the parameter file's folder),
     INPUT_CDF/<material>.cdf and INPUT_DOS/<material>.dos (or the files the
     parameter file names).  Options this library does not compute (surface
     emission, Brandt-Kitagawa ions, DSF elastic tables) are refused by the
-    library when the context is created."""
+    library when the context is created.
+
+    radiat: radiative decay times [fs] per (element, shell), which the
+    reference reads from the EADL database (Dealing_with_EADL.f90:356).  With
+    radiative decay switched on in the parameter file they must be given;
+    radiat="off" keeps photon transport but no shell decays radiatively."""
     import os
     from . import abi
     root = root or os.path.dirname(os.path.abspath(parameters_path))
     p = parse_parameters(_text(parameters_path))
     cdf = os.path.join(root, "INPUT_CDF", p["CDF_file"] or p["material"] + ".cdf")
     dos = os.path.join(root, "INPUT_DOS", p["DOS_file"] or p["material"] + ".dos")
+    if p["include_photons"] and radiat is None:
+        raise ValueError("radiative decay is on in the parameter file: give the radiative decay times per shell "
+                         "(radiat=[[...], ...]; the reference reads them from the EADL database) or radiat='off'")
     mat = read_cdf(cdf, layer=p["layer"], dos=read_dos(dos), cut_off=p["cut_off"], hole_mass=p["hole_mass"],
-                   el_eff_mass=p["El_eff_mass"])
+                   el_eff_mass=p["El_eff_mass"], radiat=None if isinstance(radiat, str) else radiat)
     mat.temp = p["temp"]
     mat.work_function = p["work_function"]
     ion = abi.Ion()
