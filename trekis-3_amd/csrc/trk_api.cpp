@@ -284,8 +284,8 @@ int trk_time_nodes(const trk_ctx *c, int32_t *Nt_out)
     return 0;
 }
 
-int trk_run_mc(trk_ctx *c, int64_t nmc, uint64_t seed, int64_t first_iter, const double *Out_R, const double *Out_V,
-               int32_t NR, trk_tallies *out)
+static int check_run_args(trk_ctx *c, int64_t nmc, int64_t first_iter, const double *Out_R, const double *Out_V,
+                          int32_t NR, const trk_tallies *out)
 {
     if (!c) return TRK_ERR_ARG;
     if (!c->tables_ready) return set_error(c, TRK_ERR_STATE, "trk_build_tables must run before trk_run_mc");
@@ -293,12 +293,28 @@ int trk_run_mc(trk_ctx *c, int64_t nmc, uint64_t seed, int64_t first_iter, const
         return set_error(c, TRK_ERR_ARG, "invalid trk_run_mc arguments");
     if (out->Nt != c->hm.Nt) return set_error(c, TRK_ERR_ARG, "tallies have Nt=%d, the run has %d time nodes", out->Nt, c->hm.Nt);
     if (out->NR != NR) return set_error(c, TRK_ERR_ARG, "tallies have NR=%d, the radial grid has %d bins", out->NR, NR);
+    if (out->Out_nh || out->Out_Eh || out->Out_Ehkin || out->Out_E_h) {
+        int ms = 0;
+        for (int a = 0; a < c->hm.m.n_atoms; ++a) ms = c->hm.m.n_shells[a] > ms ? c->hm.m.n_shells[a] : ms;
+        if (out->max_shells < ms)
+            return set_error(c, TRK_ERR_ARG, "tallies have max_shells=%d, an atom of the material has %d shells",
+                             out->max_shells, ms);
+    }
     if (out->Out_Eh_vs_E && out->n_dos != c->hm.m.n_dos)
         return set_error(c, TRK_ERR_ARG, "tallies have n_dos=%d, the material's DOS has %d points", out->n_dos, c->hm.m.n_dos);
     for (int j = 1; j < NR; ++j)
         if (!(Out_R[j] > Out_R[j - 1])) return set_error(c, TRK_ERR_ARG, "Out_R must be strictly increasing");
-    return run_mc(c, nmc, seed, first_iter, Out_R, Out_V, NR, out);
+    return 0;
 }
+
+int trk_run_mc(trk_ctx *c, int64_t nmc, uint64_t seed, int64_t first_iter, const double *Out_R, const double *Out_V,
+               int32_t NR, trk_tallies *out)
+{
+    if (!c) return TRK_ERR_ARG;
+    const int rc = check_run_args(c, nmc, first_iter, Out_R, Out_V, NR, out);
+    return rc ? rc : run_mc(c, nmc, seed, first_iter, Out_R, Out_V, NR, out);
+}
+
 
 int trk_last_timing(const trk_ctx *c, double *transport_s, double *total_s, int64_t *launches)
 {

@@ -131,4 +131,20 @@ TRK_HD double define_dE(int n, double E, double E0_min, double E0_max, double dE
     return (dE > dE_min) ? dE : dE_min;
 }
 
+// Next_free_path_2d (MC:1874-1899) in the trk namespace, for host code (the
+// event code compiles its own copy in each of its namespaces)
+TRK_HD double nfp_2d_host(const double *Ea, const double *La, int n, double E)
+{
+    if (n < 1) return 1.0e30;      // no table (no photons): an infinite path
+    int N = find_mono_2d(Ea, n, E);
+    if (N < 0) return 1.0e30;
+    if (N == 1) {
+        double M = interpolate(1, Ea[0], Ea[1], La[0], La[1], E);
+        return (M < La[0]) ? La[0] : M;
+    }
+    int Nl = N - 1;
+    if (La[Nl - 1] >= 1.0e16) return La[Nl - 1];
+    return interpolate(5, Ea[Nl - 1], Ea[N - 1], La[Nl - 1], La[N - 1], E);
+}
+
 }  // namespace trk

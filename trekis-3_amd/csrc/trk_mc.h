@@ -120,21 +120,6 @@ extern __shared__ double lds_tab[];
 constexpr int kTransportBlockFp64 = 768;
 constexpr int kTransportBlockMx = 768;
 
-// Next_free_path_2d (MC:1874-1899), for the host driver
-TRK_HD double nfp_2d(const double *Ea, const double *La, int n, double E)
-{
-    if (n < 1) return 1.0e30;      // no table (no photons): an infinite path
-    int N = find_mono_2d(Ea, n, E);
-    if (N < 0) return 1.0e30;
-    if (N == 1) {
-        double M = interpolate(1, Ea[0], Ea[1], La[0], La[1], E);
-        return (M < La[0]) ? La[0] : M;
-    }
-    int Nl = N - 1;
-    if (La[Nl - 1] >= 1.0e16) return La[Nl - 1];
-    return interpolate(5, Ea[Nl - 1], Ea[N - 1], La[Nl - 1], La[N - 1], E);
-}
-
 // The event code is compiled twice, into namespaces fp64 (deterministic
 // FP64, replays the oracle bit for bit) and mx (mixed precision); these are
 // the kernels the host driver launches from each.

@@ -232,10 +232,12 @@ __global__ void k_time_prefix(Tally T)
 }
 
 // ---------------------------------------------------------------- host side
+// Device buffers of the transport, allocated once per context and kept
+// (grown when a run needs more).
 struct McState {
     PList L[2][4];            // [ping-pong][electrons, holes, photons, ions]
     int cap[4] = {0, 0, 0, 0};
-    int *d_counts = nullptr;  // [9]: list counts of round A [4], of round B [4], transport cursor
+    int *d_counts = nullptr;  // [16]: list counts of round A [4], of round B [4], transport cursor
     int64_t max_blocks = 0;   // resident transport workgroups on the device
     int ncu = 0, max_lds = 0;
     size_t lds_bytes = 0;     // staged tables per transport workgroup
@@ -244,8 +246,9 @@ struct McState {
     unsigned long long *d_counters = nullptr;
     Tally T{};
     size_t tally_bytes = 0;
-    double *d_tim = nullptr, *d_R = nullptr;
     int NR_alloc = 0, Nt_alloc = 0, R_alloc = 0;
+    double *d_tim = nullptr, *d_R = nullptr;   // time nodes and radii of the run
+    int tim_cap = 0, R_cap = 0;
     double *d_trace = nullptr;
     unsigned long long *d_trace_n = nullptr;
     int64_t trace_cap = 0;
@@ -317,9 +320,9 @@ static int ensure_tally(trk_ctx *c, McState *s, int Nt, int NR, int nsh, int R)
     if (s->T.cnt_e && s->Nt_alloc == Nt && s->NR_alloc == NR && s->R_alloc == R && s->T.nsh == nsh) return 0;
     void *ps[] = {s->T.cnt_e, s->T.E_e, s->T.cnt_h, s->T.E_h, s->T.Ek_h, s->T.cnt_p, s->T.E_p, s->T.Elat};
     for (void *p : ps) if (p) (void)hipFree(p);
+    s->T = Tally{};
     size_t cell = (size_t)R * Nt * NR;
     Tally &T = s->T;
-    T.Nt = Nt; T.NR = NR; T.nsh = nsh; T.R = R;
     HIPC(hipMalloc((void **)&T.cnt_e, cell * 8));
     HIPC(hipMalloc((void **)&T.E_e, cell * 16));
     HIPC(hipMalloc((void **)&T.cnt_h, cell * nsh * 8));
@@ -328,6 +331,7 @@ static int ensure_tally(trk_ctx *c, McState *s, int Nt, int NR, int nsh, int R)
     HIPC(hipMalloc((void **)&T.cnt_p, cell * 8));
     HIPC(hipMalloc((void **)&T.E_p, cell * 16));
     HIPC(hipMalloc((void **)&T.Elat, cell * 16));
+    T.Nt = Nt; T.NR = NR; T.nsh = nsh; T.R = R;
     s->tally_bytes = cell * (8 + 16 + nsh * 40 + 8 + 16 + 16);
     s->Nt_alloc = Nt; s->NR_alloc = NR; s->R_alloc = R;
     return 0;
@@ -356,6 +360,27 @@ static int ensure_spec(trk_ctx *c, McState *s, int B, int Nt, int NR, int ND)
     return 0;
 }
 
+// grow a device array of doubles to hold n values
+static int ensure_doubles(trk_ctx *c, double **p, int *have, int n)
+{
+    if (*p && *have >= n) return 0;
+    if (*p) (void)hipFree(*p);
+    *p = nullptr;
+    *have = 0;
+    HIPC(hipMalloc((void **)p, sizeof(double) * (size_t)(n > 0 ? n : 1)));
+    *have = n;
+    return 0;
+}
+
+static int device_info(trk_ctx *c, McState *s)
+{
+    int dev = 0;
+    HIPC(hipGetDevice(&dev));
+    HIPC(hipDeviceGetAttribute(&s->ncu, hipDeviceAttributeMultiprocessorCount, dev));
+    HIPC(hipDeviceGetAttribute(&s->max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
+    return 0;
+}
+
 static void fill_mcdev(const trk_ctx *c, McDev &d)
 {
     const TableSet &ts = c->ts;
@@ -373,6 +398,19 @@ static void fill_mcdev(const trk_ctx *c, McDev &d)
     d.rmono_el = T[TRK_TAB_E_EL].row_mono; d.rmono_hel = T[TRK_TAB_H_EL].row_mono;
     d.nP = T[TRK_TAB_PHOT].n; d.EP = T[TRK_TAB_PHOT].E; d.LPsh = T[TRK_TAB_PHOT].L; d.phot_imfp = ts.phot_imfp;
     d.nS = T[TRK_TAB_SHI].n; d.ES = T[TRK_TAB_SHI].E; d.LSsh = T[TRK_TAB_SHI].L; d.shi_path = ts.shi_path;
+    const TableDev &td = T[TRK_TAB_SHI_DIFF];
+    d.sd_n = ts.sd_n; d.sd_off = ts.sd_off; d.sd_E = td.dE; d.sd_L = td.dL; d.sd_inv = td.dInv;
+    d.sd_nonmono = td.dNonMono; d.sd_M = ts.sd_M; d.sd_c = ts.sd_c;
+    d.lnE = ts.ln_E[TRK_TAB_E_INEL]; d.ln_el_imfp = ts.ln_el_imfp;
+    d.lnEel = ts.ln_E[TRK_TAB_E_EL]; d.ln_Lel = ts.ln_Lel;
+    d.lnEH = ts.ln_E[TRK_TAB_H_INEL]; d.ln_hole_imfp = ts.ln_hole_imfp;
+    d.lnEHel = ts.ln_E[TRK_TAB_H_EL]; d.ln_LHel = ts.ln_LHel;
+    d.ln_Lsh = ts.ln_L[TRK_TAB_E_INEL]; d.ln_LHsh = ts.ln_L[TRK_TAB_H_INEL];
+    d.ln_rhw = ts.ln_hw[TRK_TAB_E_INEL]; d.ln_rlam = ts.ln_lam[TRK_TAB_E_INEL];
+    d.ln_rhw_h = ts.ln_hw[TRK_TAB_H_INEL]; d.ln_rlam_h = ts.ln_lam[TRK_TAB_H_INEL];
+    d.ln_rhw_el = ts.ln_hw[TRK_TAB_E_EL]; d.ln_rlam_el = ts.ln_lam[TRK_TAB_E_EL];
+    d.ln_rhw_hel = ts.ln_hw[TRK_TAB_H_EL]; d.ln_rlam_hel = ts.ln_lam[TRK_TAB_H_EL];
+    d.lnES = ts.ln_E[TRK_TAB_SHI]; d.ln_LSsh = ts.ln_L[TRK_TAB_SHI]; d.ln_shi_path = ts.ln_shi_path;
 }
 
 }  // namespace trk
@@ -392,46 +430,31 @@ static double fx_to_double(__int128 v)
 // ionisations is at most a few hundred long.
 constexpr int kMaxGenerations = 1 << 16;
 
-int run_mc(trk_ctx *c, int64_t nmc, uint64_t seed, int64_t first_iter, const double *Out_R, const double *Out_V,
-           int NR, trk_tallies *out)
+// Iterations per batch: the ion's expected ionisations over the layer times
+// the pairs each makes must fit one generation's lists.
+static int64_t batch_size(const trk_ctx *c, int cap, int64_t nmc)
+{
+    const double lam = c->ts.shi_path0;
+    const double est_pairs = (lam > 0.0) ? c->hm.m.Layer / lam : 1e4;
+    int64_t B = (c->hm.o.batch > 0) ? c->hm.o.batch : (int64_t)((double)cap / (3.0 * est_pairs + 64.0));
+    if (B < 1) B = 1;
+    if (B > nmc) B = nmc;
+    return B;
+}
+
+// The run itself on buffers the caller of this function provides in s.
+static int run_core(trk_ctx *c, McState *s, int64_t nmc, uint64_t seed, int64_t first_iter, const double *Out_R,
+                    const double *Out_V, int NR, trk_tallies *out, int64_t B, bool want_spec)
 {
     auto tstart = std::chrono::steady_clock::now();
     const bool mixed = c->hm.o.precision == 1;   // FP32 elementary functions in the event code
     const HostMat &h = c->hm;
-    if (!c->mc) c->mc = new McState();
-    McState *s = c->mc;
     const int Nt = h.Nt, nsh = h.nsh;
-    const int R = (c->hm.o.tally_replicas > 0) ? c->hm.o.tally_replicas : 8;
-    // capacities: particles of each kind per round
-    int cap = (c->hm.o.gen_capacity > 0) ? c->hm.o.gen_capacity : (1 << 22);
-    if (s->cap[0] != cap) {
-        for (int g = 0; g < 2; ++g)
-            for (int k = 0; k < 3; ++k) free_list(s->L[g][k]);
-        const int caps[3] = {cap, cap, cap / 4};
-        for (int g = 0; g < 2; ++g)
-            for (int k = 0; k < 3; ++k)
-                if (alloc_list(c, s->L[g][k], caps[k], k == K_HO)) return TRK_ERR_DEVICE;
-        for (int k = 0; k < 3; ++k) s->cap[k] = caps[k];
-        if (!s->d_counts) HIPC(hipMalloc((void **)&s->d_counts, 16 * sizeof(int)));
-        if (!s->d_overflow) HIPC(hipMalloc((void **)&s->d_overflow, sizeof(int)));
-        if (!s->d_counters) HIPC(hipMalloc((void **)&s->d_counters, 16 * sizeof(unsigned long long)));
-        if (!s->ev0) { HIPC(hipEventCreate(&s->ev0)); HIPC(hipEventCreate(&s->ev1)); }
-        int per_cu = 0, dev = 0, ncu = 0;
-        HIPC(hipGetDevice(&dev));
-        HIPC(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
-        s->ncu = ncu;
-        HIPC(hipDeviceGetAttribute(&s->max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
-        (void)per_cu;
-    }
-    if (ensure_tally(c, s, Nt, NR, nsh, R)) return TRK_ERR_DEVICE;
-    if (s->d_tim) (void)hipFree(s->d_tim);
-    if (s->d_R) (void)hipFree(s->d_R);
-    HIPC(hipMalloc((void **)&s->d_tim, sizeof(double) * Nt));
-    HIPC(hipMalloc((void **)&s->d_R, sizeof(double) * NR));
     hipStream_t st = c->stream;
     HIPC(hipMemcpyAsync(s->d_tim, h.tim_glob.data(), sizeof(double) * Nt, hipMemcpyHostToDevice, st));
     HIPC(hipMemcpyAsync(s->d_R, Out_R, sizeof(double) * NR, hipMemcpyHostToDevice, st));
     Tally &T = s->T;
+    const int R = T.R;
     size_t cell = (size_t)R * Nt * NR;
     HIPC(hipMemsetAsync(T.cnt_e, 0, cell * 8, st));
     HIPC(hipMemsetAsync(T.E_e, 0, cell * 16, st));
@@ -444,28 +467,16 @@ int run_mc(trk_ctx *c, int64_t nmc, uint64_t seed, int64_t first_iter, const dou
     HIPC(hipMemsetAsync(s->d_counters, 0, 16 * sizeof(unsigned long long), st));
     HIPC(hipMemsetAsync(s->d_overflow, 0, sizeof(int), st));
 
-    // SHI_DIFF per-shell layout
-    const TableDev &td = c->ts.tab[TRK_TAB_SHI_DIFF];
-    int *d_sdn = nullptr, *d_sdM = nullptr;
-    int64_t *d_sdoff = nullptr;
-    double *d_sdc = nullptr;
-    HIPC(hipMalloc((void **)&d_sdn, sizeof(int) * nsh));
-    HIPC(hipMalloc((void **)&d_sdoff, sizeof(int64_t) * (nsh + 1)));
-    HIPC(hipMalloc((void **)&d_sdM, sizeof(int) * 2 * nsh));
-    HIPC(hipMalloc((void **)&d_sdc, sizeof(double) * 2 * nsh));
-    HIPC(hipMemcpyAsync(d_sdn, td.dn.data(), sizeof(int) * nsh, hipMemcpyHostToDevice, st));
-    HIPC(hipMemcpyAsync(d_sdoff, td.doff.data(), sizeof(int64_t) * (nsh + 1), hipMemcpyHostToDevice, st));
-
     McDev d{};
     fill_mcdev(c, d);
-    d.sd_n = d_sdn; d.sd_off = d_sdoff; d.sd_E = td.dE; d.sd_L = td.dL; d.sd_inv = td.dInv; d.sd_nonmono = td.dNonMono;
-    // per-shell constants of the mixed path's shi_transfer, computed with its
-    // own arithmetic (the FP64 path evaluates them in every event)
-    if (mixed) {
-        mx::k_shi_consts<<<(nsh + 63) / 64, 64, 0, st>>>(d, nsh, d_sdM, d_sdc);
+    // per-shell constants of the mixed path's shi_transfer, computed once
+    // per table set with its own arithmetic (the FP64 path evaluates them in
+    // every event)
+    if (mixed && !c->ts.sd_consts) {
+        mx::k_shi_consts<<<(nsh + 63) / 64, 64, 0, st>>>(d, nsh, c->ts.sd_M, c->ts.sd_c);
         HIPC(hipGetLastError());
+        c->ts.sd_consts = true;
     }
-    d.sd_M = d_sdM; d.sd_c = d_sdc;
     d.seed = seed;
     d.Tim = h.o.Tim;
     d.T_last = h.tim_glob[Nt - 1];
@@ -474,37 +485,15 @@ int run_mc(trk_ctx *c, int64_t nmc, uint64_t seed, int64_t first_iter, const dou
     d.inv_dt = (h.o.dt_flag <= 0 && h.o.dt > 0.0) ? 1.0 / h.o.dt : 0.0;
     {
         // a particle's last bracket is tried before a search only on strictly
-        // increasing grids, where it gives the search's own result
-        d.hints = 1;
-        const int kinds[5] = {TRK_TAB_E_INEL, TRK_TAB_E_EL, TRK_TAB_H_INEL, TRK_TAB_H_EL, TRK_TAB_SHI};
-        for (int k : kinds) {
-            const TableDev &t = c->ts.tab[k];
-            if (t.n <= 0 || !t.E) continue;
-            std::vector<double> g((size_t)t.n);
-            HIPC(hipMemcpyAsync(g.data(), t.E, sizeof(double) * t.n, hipMemcpyDeviceToHost, st));
-            HIPC(hipStreamSynchronize(st));
-            for (int i = 1; i < t.n; ++i)
-                if (!(g[i - 1] < g[i])) d.hints = 0;
-        }
-        // and the time grid (a flight with no node in (t0, tn] is recognised
-        // from the first node after t0) and the radii Out_R
+        // increasing grids, where it gives the search's own result: the
+        // energy grids (checked when the tables were built), the time grid
+        // (a flight with no node in (t0, tn] is recognised from the first
+        // node after t0) and the radii Out_R
+        d.hints = c->ts.grids_increasing ? 1 : 0;
         for (int i = 1; i < Nt; ++i)
             if (!(h.tim_glob[i - 1] < h.tim_glob[i])) d.hints = 0;
         for (int i = 1; i < NR; ++i)
             if (!(Out_R[i - 1] < Out_R[i])) d.hints = 0;
-    }
-    {
-        const TableSet &ts = c->ts;
-        d.lnE = ts.ln_E[TRK_TAB_E_INEL]; d.ln_el_imfp = ts.ln_el_imfp;
-        d.lnEel = ts.ln_E[TRK_TAB_E_EL]; d.ln_Lel = ts.ln_Lel;
-        d.lnEH = ts.ln_E[TRK_TAB_H_INEL]; d.ln_hole_imfp = ts.ln_hole_imfp;
-        d.lnEHel = ts.ln_E[TRK_TAB_H_EL]; d.ln_LHel = ts.ln_LHel;
-        d.ln_Lsh = ts.ln_L[TRK_TAB_E_INEL]; d.ln_LHsh = ts.ln_L[TRK_TAB_H_INEL];
-        d.ln_rhw = ts.ln_hw[TRK_TAB_E_INEL]; d.ln_rlam = ts.ln_lam[TRK_TAB_E_INEL];
-        d.ln_rhw_h = ts.ln_hw[TRK_TAB_H_INEL]; d.ln_rlam_h = ts.ln_lam[TRK_TAB_H_INEL];
-        d.ln_rhw_el = ts.ln_hw[TRK_TAB_E_EL]; d.ln_rlam_el = ts.ln_lam[TRK_TAB_E_EL];
-        d.ln_rhw_hel = ts.ln_hw[TRK_TAB_H_EL]; d.ln_rlam_hel = ts.ln_lam[TRK_TAB_H_EL];
-        d.lnES = ts.ln_E[TRK_TAB_SHI]; d.ln_LSsh = ts.ln_L[TRK_TAB_SHI]; d.ln_shi_path = ts.ln_shi_path;
     }
     d.NR = NR; d.R = s->d_R;
     d.include_photons = h.o.include_photons;
@@ -512,44 +501,15 @@ int run_mc(trk_ctx *c, int64_t nmc, uint64_t seed, int64_t first_iter, const dou
     d.counters = s->d_counters;
     d.T = T;
     if (c->trace_host && c->trace_cap > 0) {
-        if (s->trace_cap < c->trace_cap) {
-            if (s->d_trace) (void)hipFree(s->d_trace);
-            HIPC(hipMalloc((void **)&s->d_trace, sizeof(double) * 12 * c->trace_cap));
-            if (!s->d_trace_n) HIPC(hipMalloc((void **)&s->d_trace_n, sizeof(unsigned long long)));
-            s->trace_cap = c->trace_cap;
-        }
         HIPC(hipMemsetAsync(s->d_trace_n, 0, sizeof(unsigned long long), st));
         d.trace = s->d_trace; d.trace_n = s->d_trace_n; d.trace_cap = c->trace_cap;
-    }
-
-    // batch size: SHI events per iteration x pairs per SHI event must fit one generation
-    double est_pairs = 0.0;
-    {
-        std::vector<double> sp(c->ts.tab[TRK_TAB_SHI].n);
-        HIPC(hipMemcpy(sp.data(), c->ts.shi_path, sizeof(double) * sp.size(), hipMemcpyDeviceToHost));
-        std::vector<double> sE(c->ts.tab[TRK_TAB_SHI].n);
-        HIPC(hipMemcpy(sE.data(), c->ts.tab[TRK_TAB_SHI].E, sizeof(double) * sE.size(), hipMemcpyDeviceToHost));
-        double lam = nfp_2d(sE.data(), sp.data(), (int)sp.size(), h.ion.E);
-        est_pairs = (lam > 0.0) ? h.m.Layer / lam : 1e4;
-    }
-    int64_t B = (c->hm.o.batch > 0) ? c->hm.o.batch : (int64_t)((double)cap / (3.0 * est_pairs + 64.0));
-    if (B < 1) B = 1;
-    if (B > nmc) B = nmc;
-    if (s->cap[K_ION] < B) {
-        for (int g = 0; g < 2; ++g) {
-            free_list(s->L[g][K_ION]);
-            if (alloc_list(c, s->L[g][K_ION], (int)B, 1)) return TRK_ERR_DEVICE;
-        }
-        s->cap[K_ION] = (int)B;
     }
     for (int k = 0; k < 4; ++k) d.cap[k] = s->cap[k];
 
     // per-iteration normalised tallies, when the caller asks for any of them
-    const bool want_spec = out->Out_Ee_vs_E || out->Out_Eh_vs_E || out->Out_theta || out->Out_theta_h || out->Out_diff_coeff;
     const int ND = h.m.n_dos;
     d.S = Spec{};
     if (want_spec) {
-        if (ensure_spec(c, s, (int)B, Nt, NR, ND)) return TRK_ERR_DEVICE;
         HIPC(hipMemsetAsync(s->acc_eve, 0, (size_t)Nt * NR * sizeof(double), st));
         HIPC(hipMemsetAsync(s->acc_ehve, 0, (size_t)Nt * (ND > 1 ? ND : 1) * sizeof(double), st));
         HIPC(hipMemsetAsync(s->acc_th, 0, (size_t)(Nt + 1) * 180 * sizeof(double), st));
@@ -605,11 +565,12 @@ int run_mc(trk_ctx *c, int64_t nmc, uint64_t seed, int64_t first_iter, const dou
 
     double transport_s = 0.0;
     int64_t launches = 0;
-    std::vector<int> hc(8);
+    int hc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     const bool verbose = getenv("TRK_VERBOSE") != nullptr;   // per-round list sizes on stderr
     for (int64_t b0 = 0; b0 < nmc; b0 += B) {
         int nb = (int)((nmc - b0 < B) ? nmc - b0 : B);
         int cur = 0, rounds = 0;
+        auto tb0 = std::chrono::steady_clock::now();
         if (want_spec) {
             const size_t bt = (size_t)nb * Nt;
             HIPC(hipMemsetAsync(s->S.nel, 0, bt * sizeof(int), st));
@@ -624,7 +585,7 @@ int run_mc(trk_ctx *c, int64_t nmc, uint64_t seed, int64_t first_iter, const dou
             d.S.iter0 = (int)(first_iter + b0);
         }
         for (int k = 0; k < 4; ++k) d.in[k] = s->L[cur][k];
-        HIPC(hipEventRecord(s->ev0, st));
+        if (s->ev0) HIPC(hipEventRecord(s->ev0, st));
         if (c->prof) HIPC(hipEventRecord(c->pe[0], st));
         if (mixed) mx::k_ion_init<<<(nb + 63) / 64, 64, 0, st>>>(d, nb, (uint32_t)(first_iter + b0));
         else fp64::k_ion_init<<<(nb + 63) / 64, 64, 0, st>>>(d, nb, (uint32_t)(first_iter + b0));
@@ -635,7 +596,7 @@ int run_mc(trk_ctx *c, int64_t nmc, uint64_t seed, int64_t first_iter, const dou
         int nin[4] = {0, 0, 0, nb};
         for (;;) {
             if (rounds > 0) {
-                HIPC(hipMemcpyAsync(hc.data(), s->d_counts + 4 * cur, 4 * sizeof(int), hipMemcpyDeviceToHost, st));
+                HIPC(hipMemcpyAsync(hc, s->d_counts + 4 * cur, 4 * sizeof(int), hipMemcpyDeviceToHost, st));
             }
             HIPC(hipStreamSynchronize(st));
             if (c->prof && pending >= 0) {
@@ -711,23 +672,23 @@ int run_mc(trk_ctx *c, int64_t nmc, uint64_t seed, int64_t first_iter, const dou
             k_diff_coeff<<<blk(Nt), 256, 0, st>>>(s->acc_dc, S.dcs, S.dcn, nb, Nt);
             HIPC(hipGetLastError());
         }
-        HIPC(hipEventRecord(s->ev1, st));
-        HIPC(hipEventSynchronize(s->ev1));
-        float ms = 0.f;
-        HIPC(hipEventElapsedTime(&ms, s->ev0, s->ev1));
-        transport_s += ms * 1e-3;
+        if (s->ev1) {
+            HIPC(hipEventRecord(s->ev1, st));
+            HIPC(hipEventSynchronize(s->ev1));
+            float ms = 0.f;
+            HIPC(hipEventElapsedTime(&ms, s->ev0, s->ev1));
+            transport_s += ms * 1e-3;
+        } else {
+            HIPC(hipStreamSynchronize(st));
+            transport_s += std::chrono::duration<double>(std::chrono::steady_clock::now() - tb0).count();
+        }
     }
     HIPC(hipStreamSynchronize(st));
-    (void)hipFree(d_sdn);
-    (void)hipFree(d_sdoff);
-    (void)hipFree(d_sdM);
-    (void)hipFree(d_sdc);
 
     // counters and trace
     unsigned long long ctr[16];
     HIPC(hipMemcpy(ctr, s->d_counters, sizeof(ctr), hipMemcpyDeviceToHost));
-    int64_t tot_ev = 0;
-    for (int k = 0; k < 8; ++k) { out->n_events[k] += (int64_t)ctr[k]; tot_ev += (int64_t)ctr[k]; }
+    for (int k = 0; k < 8; ++k) out->n_events[k] += (int64_t)ctr[k];
     if (c->trace_host && c->trace_cap > 0) {
         unsigned long long tn = 0;
         HIPC(hipMemcpy(&tn, s->d_trace_n, sizeof(tn), hipMemcpyDeviceToHost));
@@ -823,8 +784,62 @@ int run_mc(trk_ctx *c, int64_t nmc, uint64_t seed, int64_t first_iter, const dou
     c->last_transport_s = transport_s;
     c->last_total_s = std::chrono::duration<double>(std::chrono::steady_clock::now() - tstart).count();
     c->last_launches = launches;
-    (void)tot_ev;
     return 0;
 }
+
+// do_Monte_Carlo on buffers this context allocates on its first run and
+// keeps for the next ones.
+int run_mc(trk_ctx *c, int64_t nmc, uint64_t seed, int64_t first_iter, const double *Out_R, const double *Out_V,
+           int NR, trk_tallies *out)
+{
+    const HostMat &h = c->hm;
+    if (!c->mc) c->mc = new McState();
+    McState *s = c->mc;
+    const int Nt = h.Nt, nsh = h.nsh;
+    const int R = (h.o.tally_replicas > 0) ? h.o.tally_replicas : 8;
+    // capacities: particles of each kind per round
+    const int cap = (h.o.gen_capacity > 0) ? h.o.gen_capacity : (1 << 22);
+    if (s->cap[0] != cap) {
+        for (int g = 0; g < 2; ++g)
+            for (int k = 0; k < 3; ++k) free_list(s->L[g][k]);
+        s->cap[0] = s->cap[1] = s->cap[2] = 0;
+        const int caps[3] = {cap, cap, cap / 4};
+        for (int g = 0; g < 2; ++g)
+            for (int k = 0; k < 3; ++k)
+                if (alloc_list(c, s->L[g][k], caps[k], k == K_HO)) return TRK_ERR_DEVICE;
+        for (int k = 0; k < 3; ++k) s->cap[k] = caps[k];
+    }
+    if (!s->d_counts) HIPC(hipMalloc((void **)&s->d_counts, 16 * sizeof(int)));
+    if (!s->d_overflow) HIPC(hipMalloc((void **)&s->d_overflow, sizeof(int)));
+    if (!s->d_counters) HIPC(hipMalloc((void **)&s->d_counters, 16 * sizeof(unsigned long long)));
+    if (!s->ev0) { HIPC(hipEventCreate(&s->ev0)); HIPC(hipEventCreate(&s->ev1)); }
+    if (!s->ncu && device_info(c, s)) return TRK_ERR_DEVICE;
+    if (ensure_tally(c, s, Nt, NR, nsh, R)) return TRK_ERR_DEVICE;
+    if (ensure_doubles(c, &s->d_tim, &s->tim_cap, Nt) || ensure_doubles(c, &s->d_R, &s->R_cap, NR))
+        return TRK_ERR_DEVICE;
+    const int64_t B = batch_size(c, cap, nmc);
+    if (s->cap[K_ION] < B) {
+        for (int g = 0; g < 2; ++g) {
+            free_list(s->L[g][K_ION]);
+            s->cap[K_ION] = 0;
+            if (alloc_list(c, s->L[g][K_ION], (int)B, 1)) return TRK_ERR_DEVICE;
+        }
+        s->cap[K_ION] = (int)B;
+    }
+    const bool want_spec = out->Out_Ee_vs_E || out->Out_Eh_vs_E || out->Out_theta || out->Out_theta_h || out->Out_diff_coeff;
+    if (want_spec && ensure_spec(c, s, (int)B, Nt, NR, h.m.n_dos)) return TRK_ERR_DEVICE;
+    if (c->trace_host && c->trace_cap > 0) {
+        if (s->trace_cap < c->trace_cap) {
+            if (s->d_trace) (void)hipFree(s->d_trace);
+            s->d_trace = nullptr;
+            s->trace_cap = 0;
+            HIPC(hipMalloc((void **)&s->d_trace, sizeof(double) * 12 * c->trace_cap));
+            s->trace_cap = c->trace_cap;
+        }
+        if (!s->d_trace_n) HIPC(hipMalloc((void **)&s->d_trace_n, sizeof(unsigned long long)));
+    }
+    return run_core(c, s, nmc, seed, first_iter, Out_R, Out_V, NR, out, B, want_spec);
+}
+
 
 }  // namespace trk

@@ -73,6 +73,18 @@ struct TableSet {
     // kinds) and of the cumulative rows' transfer energies and lambdas
     // (same layout as hw / lam)
     double *ln_L[7] = {}, *ln_hw[7] = {}, *ln_lam[7] = {};
+    // the material kinds (electron/hole inelastic and elastic, photon) and
+    // their derived arrays belong to another table set (an ion context built
+    // on a material context): free_tables leaves them alone
+    bool shared_material = false;
+    // run constants fixed by the tables, set at the end of build_tables
+    bool grids_increasing = false;  // electron, hole and ion energy grids strictly increasing
+    double shi_path0 = 0.0;         // the ion's fused mean free path at its initial energy [A]
+    int *sd_n = nullptr;            // [nsh] SHI_DIFF row length per shell (device)
+    int64_t *sd_off = nullptr;      // [nsh+1] SHI_DIFF row offsets (device)
+    int *sd_M = nullptr;            // [2*nsh] mixed-path transfer constants (k_shi_consts)
+    double *sd_c = nullptr;         // [2*nsh]
+    bool sd_consts = false;         // sd_M / sd_c filled
     int64_t launches = 0;
     double build_seconds = 0.0;
     int64_t dcs_evals = 0;         // number of q-integrals evaluated
@@ -80,7 +92,10 @@ struct TableSet {
 
 struct HostMat;  // defined in trk_ctx.h
 
-int build_tables(const HostMat &hm, const MatDev &md, TableSet &ts, hipStream_t st, std::string &err);
+// mat: when given, the material kinds are taken from that (built) table set
+// and only the ion's tables are computed
+int build_tables(const HostMat &hm, const MatDev &md, TableSet &ts, hipStream_t st, std::string &err,
+                 const TableSet *mat = nullptr);
 void free_tables(TableSet &ts);
 
 }  // namespace trk

@@ -633,6 +633,16 @@ static inline unsigned nblk(int64_t n, int b) { return (unsigned)((n + b - 1) / 
 
 void free_tables(TableSet &ts)
 {
+    if (ts.shared_material) {
+        // forget the borrowed material arrays; free only the ion's own
+        for (int k : {TRK_TAB_E_INEL, TRK_TAB_H_INEL, TRK_TAB_E_EL, TRK_TAB_H_EL, TRK_TAB_PHOT}) {
+            ts.tab[k] = TableDev();
+            ts.ln_E[k] = ts.ln_L[k] = ts.ln_hw[k] = ts.ln_lam[k] = nullptr;
+        }
+        ts.el_imfp = ts.hole_imfp = ts.phot_imfp = nullptr;
+        ts.ln_el_imfp = ts.ln_hole_imfp = ts.ln_Lel = ts.ln_LHel = nullptr;
+        ts.shared_material = false;
+    }
     for (auto &t : ts.tab) {
         if (t.E) (void)hipFree(t.E);
         if (t.L) (void)hipFree(t.L);
@@ -657,6 +667,11 @@ void free_tables(TableSet &ts)
         for (double **p : {&ts.ln_L[k], &ts.ln_hw[k], &ts.ln_lam[k]}) { if (*p) (void)hipFree(*p); *p = nullptr; }
     double **lp[] = {&ts.ln_el_imfp, &ts.ln_hole_imfp, &ts.ln_Lel, &ts.ln_LHel, &ts.ln_shi_path};
     for (double **p : lp) { if (*p) (void)hipFree(*p); *p = nullptr; }
+    void *sp[] = {ts.sd_n, ts.sd_off, ts.sd_M, ts.sd_c};
+    for (void *p : sp) if (p) (void)hipFree(p);
+    ts.sd_n = nullptr; ts.sd_off = nullptr; ts.sd_M = nullptr; ts.sd_c = nullptr;
+    ts.sd_consts = false;
+    ts.grids_increasing = false;
 }
 
 static int upload_grid(TableDev &t, const std::vector<double> &g, int nsh, std::string &err)
@@ -671,10 +686,24 @@ static int upload_grid(TableDev &t, const std::vector<double> &g, int nsh, std::
     return 0;
 }
 
-int build_tables(const HostMat &h, const MatDev &md, TableSet &ts, hipStream_t st, std::string &err)
+// The material kinds of a built table set, borrowed by an ion's table set.
+static void borrow_material(TableSet &ts, const TableSet &mat)
+{
+    for (int k : {TRK_TAB_E_INEL, TRK_TAB_H_INEL, TRK_TAB_E_EL, TRK_TAB_H_EL, TRK_TAB_PHOT}) {
+        ts.tab[k] = mat.tab[k];
+        ts.ln_E[k] = mat.ln_E[k]; ts.ln_L[k] = mat.ln_L[k]; ts.ln_hw[k] = mat.ln_hw[k]; ts.ln_lam[k] = mat.ln_lam[k];
+    }
+    ts.el_imfp = mat.el_imfp; ts.hole_imfp = mat.hole_imfp; ts.phot_imfp = mat.phot_imfp;
+    ts.ln_el_imfp = mat.ln_el_imfp; ts.ln_hole_imfp = mat.ln_hole_imfp; ts.ln_Lel = mat.ln_Lel; ts.ln_LHel = mat.ln_LHel;
+    ts.shared_material = true;
+}
+
+int build_tables(const HostMat &h, const MatDev &md, TableSet &ts, hipStream_t st, std::string &err,
+                 const TableSet *mat)
 {
     auto t_start = std::chrono::steady_clock::now();
     free_tables(ts);
+    const bool own = (mat == nullptr);     // build the material kinds too
     const int nsh = h.nsh;
     double dosmax = h.dosE.empty() ? 0.0 : h.dosE[0];
     for (double v : h.dosE) dosmax = (v > dosmax) ? v : dosmax;
@@ -694,32 +723,40 @@ int build_tables(const HostMat &h, const MatDev &md, TableSet &ts, hipStream_t s
     }
     TableDev *T = ts.tab;
     int rc;
-    if ((rc = upload_grid(T[TRK_TAB_E_INEL], gE, nsh, err))) return rc;
-    if ((rc = upload_grid(T[TRK_TAB_H_INEL], gH, nsh, err))) return rc;
-    if ((rc = upload_grid(T[TRK_TAB_E_EL], gEel, 1, err))) return rc;
-    if ((rc = upload_grid(T[TRK_TAB_H_EL], gHel, 1, err))) return rc;
-    if (!gP.empty() && (rc = upload_grid(T[TRK_TAB_PHOT], gP, nsh, err))) return rc;
+    if (own) {
+        if ((rc = upload_grid(T[TRK_TAB_E_INEL], gE, nsh, err))) return rc;
+        if ((rc = upload_grid(T[TRK_TAB_H_INEL], gH, nsh, err))) return rc;
+        if ((rc = upload_grid(T[TRK_TAB_E_EL], gEel, 1, err))) return rc;
+        if ((rc = upload_grid(T[TRK_TAB_H_EL], gHel, 1, err))) return rc;
+        if (!gP.empty() && (rc = upload_grid(T[TRK_TAB_PHOT], gP, nsh, err))) return rc;
+    } else {
+        borrow_material(ts, *mat);
+    }
     if ((rc = upload_grid(T[TRK_TAB_SHI], gS, nsh, err))) return rc;
     T[TRK_TAB_SHI_DIFF].nsh = nsh;
 
     // row jobs
     std::vector<RowJob> jobs;
-    for (int i = 0; i < (int)gE.size(); ++i)
-        for (int s = 0; s < nsh; ++s) jobs.push_back(inel_job(h, gE[i], s, 0, 1, TRK_TAB_E_INEL, i));
-    for (int i = 0; i < (int)gH.size(); ++i)
-        for (int s = 0; s < nsh; ++s) jobs.push_back(inel_job(h, gH[i], s, 1, s == h.vb, TRK_TAB_H_INEL, i));
-    if (h.o.kind_of_EMFP == 1) {
-        for (int i = 0; i < (int)gEel.size(); ++i) jobs.push_back(phonon_job(h, gEel[i], 0, TRK_TAB_E_EL, i));
-        for (int i = 0; i < (int)gHel.size(); ++i) jobs.push_back(phonon_job(h, gHel[i], 1, TRK_TAB_H_EL, i));
+    if (own) {
+        for (int i = 0; i < (int)gE.size(); ++i)
+            for (int s = 0; s < nsh; ++s) jobs.push_back(inel_job(h, gE[i], s, 0, 1, TRK_TAB_E_INEL, i));
+        for (int i = 0; i < (int)gH.size(); ++i)
+            for (int s = 0; s < nsh; ++s) jobs.push_back(inel_job(h, gH[i], s, 1, s == h.vb, TRK_TAB_H_INEL, i));
+        if (h.o.kind_of_EMFP == 1) {
+            for (int i = 0; i < (int)gEel.size(); ++i) jobs.push_back(phonon_job(h, gEel[i], 0, TRK_TAB_E_EL, i));
+            for (int i = 0; i < (int)gHel.size(); ++i) jobs.push_back(phonon_job(h, gHel[i], 1, TRK_TAB_H_EL, i));
+        }
     }
     for (int i = 0; i < (int)gS.size(); ++i)
         for (int s = 0; s < nsh; ++s) jobs.push_back(shi_job(h, gS[i], s, 0, i));
     for (int s = 0; s < nsh; ++s) jobs.push_back(shi_job(h, h.ion.E, s, 1, 0));
-    T[TRK_TAB_E_INEL].has_rows.assign(nsh, 1);
-    T[TRK_TAB_H_INEL].has_rows[h.vb] = 1;
-    if (h.o.kind_of_EMFP == 1) {
-        T[TRK_TAB_E_EL].has_rows[0] = 1;
-        T[TRK_TAB_H_EL].has_rows[0] = 1;
+    if (own) {
+        T[TRK_TAB_E_INEL].has_rows.assign(nsh, 1);
+        T[TRK_TAB_H_INEL].has_rows[h.vb] = 1;
+        if (h.o.kind_of_EMFP == 1) {
+            T[TRK_TAB_E_EL].has_rows[0] = 1;
+            T[TRK_TAB_H_EL].has_rows[0] = 1;
+        }
     }
     const int nrows = (int)jobs.size();
 
@@ -746,12 +783,14 @@ int build_tables(const HostMat &h, const MatDev &md, TableSet &ts, hipStream_t s
     const int64_t nsteps = step_off[nrows];
 
     // row storage per kind: row_off over (shell, ie)
-    for (int kind : {TRK_TAB_E_INEL, TRK_TAB_H_INEL, TRK_TAB_E_EL, TRK_TAB_H_EL}) {
-        TableDev &t = T[kind];
-        t.row_off_h.assign((size_t)t.nsh * t.n + 1, 0);
-    }
+    if (own)
+        for (int kind : {TRK_TAB_E_INEL, TRK_TAB_H_INEL, TRK_TAB_E_EL, TRK_TAB_H_EL}) {
+            TableDev &t = T[kind];
+            t.row_off_h.assign((size_t)t.nsh * t.n + 1, 0);
+        }
     std::vector<int64_t> rowlen_by_slot[4];
-    for (int kind = 0; kind < 4; ++kind) rowlen_by_slot[kind].assign((size_t)T[kind].nsh * T[kind].n, 0);
+    if (own)
+        for (int kind = 0; kind < 4; ++kind) rowlen_by_slot[kind].assign((size_t)T[kind].nsh * T[kind].n, 0);
     std::vector<int> dn(nsh, 0);
     for (int r = 0; r < nrows; ++r) {
         const RowJob &j = jobs[r];
@@ -759,7 +798,7 @@ int build_tables(const HostMat &h, const MatDev &md, TableSet &ts, hipStream_t s
         if (!j.with_row) continue;
         rowlen_by_slot[j.kind][(size_t)j.shell * T[j.kind].n + j.ie] = len[r];
     }
-    for (int kind = 0; kind < 4; ++kind) {
+    for (int kind = 0; kind < 4 && own; ++kind) {
         TableDev &t = T[kind];
         for (size_t i = 0; i < rowlen_by_slot[kind].size(); ++i) t.row_off_h[i + 1] = t.row_off_h[i] + rowlen_by_slot[kind][i];
         int64_t tot = t.row_off_h.back();
@@ -818,42 +857,45 @@ int build_tables(const HostMat &h, const MatDev &md, TableSet &ts, hipStream_t s
     HIP_OK(hipGetLastError());
 
     // Mott / disabled elastic tables
-    if (h.o.kind_of_EMFP != 1) {
+    if (own && h.o.kind_of_EMFP != 1) {
         k_mott<<<nblk(T[TRK_TAB_E_EL].n, 128), 128, 0, st>>>(md, T[TRK_TAB_E_EL].E, T[TRK_TAB_E_EL].n, 0,
                                                               T[TRK_TAB_E_EL].L, T[TRK_TAB_E_EL].dEdx);
         k_mott<<<nblk(T[TRK_TAB_H_EL].n, 128), 128, 0, st>>>(md, T[TRK_TAB_H_EL].E, T[TRK_TAB_H_EL].n, 1,
                                                               T[TRK_TAB_H_EL].L, T[TRK_TAB_H_EL].dEdx);
         HIP_OK(hipGetLastError());
     }
-    if (!gP.empty()) {
+    if (own && !gP.empty()) {
         k_phot<<<nblk((int64_t)T[TRK_TAB_PHOT].n * nsh, 128), 128, 0, st>>>(md, T[TRK_TAB_PHOT].E, T[TRK_TAB_PHOT].n,
                                                                              T[TRK_TAB_PHOT].L, T[TRK_TAB_PHOT].dEdx);
         HIP_OK(hipGetLastError());
     }
     // fused totals
-    if (dalloc(&ts.el_imfp, T[TRK_TAB_E_INEL].n, err) || dalloc(&ts.hole_imfp, T[TRK_TAB_H_INEL].n, err) ||
-        dalloc(&ts.shi_path, T[TRK_TAB_SHI].n, err))
-        return TRK_ERR_DEVICE;
-    k_fuse<<<nblk(T[TRK_TAB_E_INEL].n, 128), 128, 0, st>>>(T[TRK_TAB_E_INEL].L, T[TRK_TAB_E_INEL].n, nsh, 1, ts.el_imfp);
-    k_fuse<<<nblk(T[TRK_TAB_H_INEL].n, 128), 128, 0, st>>>(T[TRK_TAB_H_INEL].L, T[TRK_TAB_H_INEL].n, nsh, 0, ts.hole_imfp);
+    if (dalloc(&ts.shi_path, T[TRK_TAB_SHI].n, err)) return TRK_ERR_DEVICE;
     k_fuse<<<nblk(T[TRK_TAB_SHI].n, 128), 128, 0, st>>>(T[TRK_TAB_SHI].L, T[TRK_TAB_SHI].n, nsh, 0, ts.shi_path);
-    if (!gP.empty()) {
+    if (own) {
+        if (dalloc(&ts.el_imfp, T[TRK_TAB_E_INEL].n, err) || dalloc(&ts.hole_imfp, T[TRK_TAB_H_INEL].n, err))
+            return TRK_ERR_DEVICE;
+        k_fuse<<<nblk(T[TRK_TAB_E_INEL].n, 128), 128, 0, st>>>(T[TRK_TAB_E_INEL].L, T[TRK_TAB_E_INEL].n, nsh, 1, ts.el_imfp);
+        k_fuse<<<nblk(T[TRK_TAB_H_INEL].n, 128), 128, 0, st>>>(T[TRK_TAB_H_INEL].L, T[TRK_TAB_H_INEL].n, nsh, 0, ts.hole_imfp);
+    }
+    if (own && !gP.empty()) {
         if (dalloc(&ts.phot_imfp, T[TRK_TAB_PHOT].n, err)) return TRK_ERR_DEVICE;
         k_fuse<<<nblk(T[TRK_TAB_PHOT].n, 128), 128, 0, st>>>(T[TRK_TAB_PHOT].L, T[TRK_TAB_PHOT].n, nsh, 0, ts.phot_imfp);
     }
     HIP_OK(hipGetLastError());
     // logs of the grids and of the tables the transport interpolates most
     for (int k = 0; k < 6; ++k) {
+        if (!own && k != TRK_TAB_SHI) continue;
         if (T[k].n <= 0 || !T[k].E) continue;
         if (dalloc(&ts.ln_E[k], T[k].n, err)) return TRK_ERR_DEVICE;
         k_log<<<nblk(T[k].n, 128), 128, 0, st>>>(T[k].E, T[k].n, ts.ln_E[k]);
     }
-    struct { const double *src; int n; double **dst; } lg[] = {
-        {ts.el_imfp, T[TRK_TAB_E_INEL].n, &ts.ln_el_imfp}, {ts.hole_imfp, T[TRK_TAB_H_INEL].n, &ts.ln_hole_imfp},
-        {T[TRK_TAB_E_EL].L, T[TRK_TAB_E_EL].n, &ts.ln_Lel}, {T[TRK_TAB_H_EL].L, T[TRK_TAB_H_EL].n, &ts.ln_LHel},
-        {ts.shi_path, T[TRK_TAB_SHI].n, &ts.ln_shi_path}};
+    struct { const double *src; int n; double **dst; bool mat; } lg[] = {
+        {ts.el_imfp, T[TRK_TAB_E_INEL].n, &ts.ln_el_imfp, true}, {ts.hole_imfp, T[TRK_TAB_H_INEL].n, &ts.ln_hole_imfp, true},
+        {T[TRK_TAB_E_EL].L, T[TRK_TAB_E_EL].n, &ts.ln_Lel, true}, {T[TRK_TAB_H_EL].L, T[TRK_TAB_H_EL].n, &ts.ln_LHel, true},
+        {ts.shi_path, T[TRK_TAB_SHI].n, &ts.ln_shi_path, false}};
     for (auto &g : lg) {
-        if (!g.src || g.n <= 0) continue;
+        if (!g.src || g.n <= 0 || (g.mat && !own)) continue;
         if (dalloc(g.dst, g.n, err)) return TRK_ERR_DEVICE;
         k_log<<<nblk(g.n, 128), 128, 0, st>>>(g.src, g.n, *g.dst);
     }
@@ -861,13 +903,13 @@ int build_tables(const HostMat &h, const MatDev &md, TableSet &ts, hipStream_t s
     // sampling): interpolation then takes one log of the argument
     for (int k : {TRK_TAB_E_INEL, TRK_TAB_H_INEL, TRK_TAB_SHI}) {
         const int64_t m = (int64_t)T[k].nsh * T[k].n;
-        if (!T[k].L || m <= 0) continue;
+        if (!T[k].L || m <= 0 || (!own && k != TRK_TAB_SHI)) continue;
         if (dalloc(&ts.ln_L[k], m, err)) return TRK_ERR_DEVICE;
         k_log<<<(unsigned)((m + 127) / 128), 128, 0, st>>>(T[k].L, m, ts.ln_L[k]);
     }
     for (int k : {TRK_TAB_E_INEL, TRK_TAB_H_INEL, TRK_TAB_E_EL, TRK_TAB_H_EL}) {
         const int64_t m = T[k].row_off_h.empty() ? 0 : T[k].row_off_h.back();
-        if (!T[k].hw || m <= 0) continue;
+        if (!own || !T[k].hw || m <= 0) continue;
         if (dalloc(&ts.ln_hw[k], m, err) || dalloc(&ts.ln_lam[k], m, err)) return TRK_ERR_DEVICE;
         k_log<<<(unsigned)((m + 127) / 128), 128, 0, st>>>(T[k].hw, m, ts.ln_hw[k]);
         k_log<<<(unsigned)((m + 127) / 128), 128, 0, st>>>(T[k].lam, m, ts.ln_lam[k]);
@@ -877,6 +919,25 @@ int build_tables(const HostMat &h, const MatDev &md, TableSet &ts, hipStream_t s
     }
     HIP_OK(hipGetLastError());
     HIP_OK(hipStreamSynchronize(st));
+    // run constants: a particle's last bracket may stand in for a search only
+    // on strictly increasing grids; the ion's first free path sizes batches
+    {
+        bool inc = own || mat->grids_increasing;
+        for (const std::vector<double> *g : {&gE, &gEel, &gH, &gHel, &gS})
+            for (size_t i = 1; i < g->size(); ++i)
+                if (!((*g)[i - 1] < (*g)[i])) inc = false;
+        ts.grids_increasing = inc;
+        std::vector<double> sp(gS.size());
+        HIP_OK(hipMemcpy(sp.data(), ts.shi_path, sizeof(double) * sp.size(), hipMemcpyDeviceToHost));
+        ts.shi_path0 = nfp_2d_host(gS.data(), sp.data(), (int)gS.size(), h.ion.E);
+        const TableDev &td = T[TRK_TAB_SHI_DIFF];
+        if (dalloc(&ts.sd_n, (size_t)nsh, err) || dalloc(&ts.sd_off, (size_t)nsh + 1, err) ||
+            dalloc(&ts.sd_M, (size_t)2 * nsh, err) || dalloc(&ts.sd_c, (size_t)2 * nsh, err))
+            return TRK_ERR_DEVICE;
+        HIP_OK(hipMemcpy(ts.sd_n, td.dn.data(), sizeof(int) * nsh, hipMemcpyHostToDevice));
+        HIP_OK(hipMemcpy(ts.sd_off, td.doff.data(), sizeof(int64_t) * (nsh + 1), hipMemcpyHostToDevice));
+        ts.sd_consts = false;
+    }
     (void)hipFree(d_jobs); (void)hipFree(d_cnt); (void)hipFree(d_len); (void)hipFree(d_doff);
     (void)hipFree(d_step_off); (void)hipFree(d_Ek); (void)hipFree(d_dEk); (void)hipFree(d_f0);
     (void)hipFree(d_fa); (void)hipFree(d_fb); (void)hipFree(d_step_row); (void)hipFree(d_outs);
