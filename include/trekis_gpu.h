@@ -198,6 +198,25 @@ int  trk_time_nodes(const trk_ctx *ctx, int32_t *Nt_out);
 int  trk_run_mc(trk_ctx *ctx, int64_t nmc, uint64_t seed, int64_t first_iter,
                 const double *Out_R, const double *Out_V, int32_t NR, trk_tallies *out);
 
+/* trk_run_mc without device allocation.  No counterpart in the reference:
+ * this entry point is for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  Same arguments and results as trk_run_mc, on a context whose
+ * tables are built (trk_ctx_create and trk_build_tables allocate; this call
+ * does not: no hipMalloc, hipFree or event creation).  Every device buffer of
+ * the run -- particle lists, one tally replica, the per-iteration histograms
+ * and the caller's Out_* arrays staged for a device-side finish -- is carved
+ * from a static device array of TRK_NOALLOC_SCRATCH_BYTES that each GPU the
+ * library runs on holds until the process ends.  The list capacity is the
+ * largest power of two up to 2^22 whose layout fits (options.gen_capacity
+ * fixes it); a run that does not fit returns TRK_ERR_CAPACITY before any
+ * launch, and event traces (trk_set_trace) return TRK_ERR_STATE.  Every call
+ * on one GPU uses that GPU's array and clears what it needs; calls that
+ * overlap on one GPU (from several threads or contexts) race and must not. */
+#define TRK_NOALLOC_SCRATCH_BYTES ((size_t)128 << 20)
+int  trk_run_mc_noalloc(trk_ctx *ctx, int64_t nmc, uint64_t seed, int64_t first_iter,
+                        const double *Out_R, const double *Out_V, int32_t NR, trk_tallies *out);
+
 /* Timing of the last trk_run_mc: seconds from the first ion launch to the end
  * of the last generation of each batch (HIP events on the library's stream),
  * wall seconds of the whole call, and kernel launches.  The reference times

@@ -10,6 +10,8 @@
 namespace trk {
 int run_mc(trk_ctx *c, int64_t nmc, uint64_t seed, int64_t first_iter, const double *Out_R, const double *Out_V,
            int NR, trk_tallies *out);
+int run_mc_noalloc(trk_ctx *c, int64_t nmc, uint64_t seed, int64_t first_iter, const double *Out_R,
+                   const double *Out_V, int NR, trk_tallies *out);
 void mc_free(McState *s);
 }  // namespace trk
 
@@ -315,6 +317,13 @@ int trk_run_mc(trk_ctx *c, int64_t nmc, uint64_t seed, int64_t first_iter, const
     return rc ? rc : run_mc(c, nmc, seed, first_iter, Out_R, Out_V, NR, out);
 }
 
+int trk_run_mc_noalloc(trk_ctx *c, int64_t nmc, uint64_t seed, int64_t first_iter, const double *Out_R,
+                       const double *Out_V, int32_t NR, trk_tallies *out)
+{
+    if (!c) return TRK_ERR_ARG;
+    const int rc = check_run_args(c, nmc, first_iter, Out_R, Out_V, NR, out);
+    return rc ? rc : run_mc_noalloc(c, nmc, seed, first_iter, Out_R, Out_V, NR, out);
+}
 
 int trk_last_timing(const trk_ctx *c, double *transport_s, double *total_s, int64_t *launches)
 {

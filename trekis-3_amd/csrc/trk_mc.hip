@@ -32,6 +32,7 @@
 #include <cmath>
 #include <cstring>
 #include <string>
+#include <mutex>
 #include <vector>
 #include "trk_mc.h"
 
@@ -442,9 +443,25 @@ static int64_t batch_size(const trk_ctx *c, int cap, int64_t nmc)
     return B;
 }
 
-// The run itself on buffers the caller of this function provides in s.
+// Buffers of the device finalize used by runs without allocation: the
+// caller's Out_* arrays staged in device memory, the radial volumes and the
+// per-node integer sums.
+struct NoAllocBufs {
+    double *stage = nullptr;
+    size_t stage_n = 0;
+    double *d_V = nullptr;
+    unsigned long long *node_sums = nullptr;   // [Nt][4]: lattice energy of the node, the rest of tot_E
+};
+
+static int finalize_on_device(trk_ctx *c, McState *s, const NoAllocBufs &na, const double *Out_V, int NR,
+                              trk_tallies *out, bool want_spec);
+
+// The run itself on buffers the caller of this function provides in s (and
+// na, for a run without allocation: the tallies are then finished on the
+// device instead of the host).
 static int run_core(trk_ctx *c, McState *s, int64_t nmc, uint64_t seed, int64_t first_iter, const double *Out_R,
-                    const double *Out_V, int NR, trk_tallies *out, int64_t B, bool want_spec)
+                    const double *Out_V, int NR, trk_tallies *out, int64_t B, bool want_spec,
+                    const NoAllocBufs *nab = nullptr)
 {
     auto tstart = std::chrono::steady_clock::now();
     const bool mixed = c->hm.o.precision == 1;   // FP32 elementary functions in the event code
@@ -706,6 +723,14 @@ static int run_core(trk_ctx *c, McState *s, int64_t nmc, uint64_t seed, int64_t 
         k_time_prefix<<<(unsigned)((ncol + 63) / 64), 64, 0, st>>>(T);
         HIPC(hipGetLastError());
     }
+    if (nab) {
+        const int frc = finalize_on_device(c, s, *nab, Out_V, NR, out, want_spec);
+        if (frc) return frc;
+        c->last_transport_s = transport_s;
+        c->last_total_s = std::chrono::duration<double>(std::chrono::steady_clock::now() - tstart).count();
+        c->last_launches = launches;
+        return 0;
+    }
     const size_t plane = (size_t)Nt * NR;
     std::vector<unsigned long long> he(plane), hEe(2 * plane), hh(plane * nsh), hEh(2 * plane * nsh),
         hEk(2 * plane * nsh), hp(plane), hEp(2 * plane), hEl(2 * plane);
@@ -841,5 +866,327 @@ int run_mc(trk_ctx *c, int64_t nmc, uint64_t seed, int64_t first_iter, const dou
     return run_core(c, s, nmc, seed, first_iter, Out_R, Out_V, NR, out, B, want_spec);
 }
 
+
+// ---------------------------------------------------------------- runs without allocation
+// Every device buffer of trk_run_mc_noalloc (particle lists, tallies,
+// per-iteration histograms, the staged Out_* arrays) is carved from this
+// array.  Each GPU the library runs on holds its own copy until the process
+// ends.  In host code its name is not a device address: the address on a
+// device comes from hipGetSymbolAddress, made once per device with that
+// device current, and only that address is used.
+static __device__ __align__(256) unsigned char scratch[TRK_NOALLOC_SCRATCH_BYTES];
+
+constexpr int kMaxDevices = 64;
+static std::mutex g_scratch_mu;
+static unsigned char *g_scratch_ptr[kMaxDevices] = {};
+
+static int scratch_base(trk_ctx *c, unsigned char **base)
+{
+    int dev = -1;
+    HIPC(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kMaxDevices) return set_error(c, TRK_ERR_DEVICE, "device %d out of range", dev);
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void *p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            return set_error(c, TRK_ERR_DEVICE, "hipGetSymbolAddress(scratch): %s", hipGetErrorString(e));
+        g_scratch_ptr[dev] = (unsigned char *)p;
+    }
+    *base = g_scratch_ptr[dev];
+    return 0;
+}
+
+// Hands out 256-byte aligned pieces of a region; with base == nullptr it
+// only counts the bytes a layout needs.
+struct Carver {
+    unsigned char *base;
+    size_t off = 0;
+    template <class T> T *take(size_t n)
+    {
+        off = (off + 255) & ~(size_t)255;
+        T *p = base ? (T *)(base + off) : nullptr;
+        off += (n ? n : 1) * sizeof(T);
+        return p;
+    }
+};
+
+static void carve_list(Carver &cv, PList &l, size_t n, int hole)
+{
+    l.E = cv.take<double>(n); l.t0 = cv.take<double>(n); l.tn = cv.take<double>(n);
+    l.X = cv.take<double>(n); l.Y = cv.take<double>(n); l.Z = cv.take<double>(n);
+    l.L = cv.take<double>(n); l.th = cv.take<double>(n); l.ph = cv.take<double>(n);
+    if (hole) { l.Ehkin = cv.take<double>(n); l.Mass = cv.take<double>(n); l.sh = cv.take<int>(n); }
+    else { l.Ehkin = l.Mass = nullptr; l.sh = nullptr; }
+    l.id = cv.take<uint64_t>(n); l.nd = cv.take<uint32_t>(n); l.ne = cv.take<uint32_t>(n); l.iter = cv.take<uint32_t>(n);
+}
+
+// doubles of the caller's Out_* arrays the device finalize stages
+static size_t stage_doubles(const trk_ctx *c, const trk_tallies *out, int NR, bool want_spec)
+{
+    const size_t Nt = (size_t)c->hm.Nt, na = (size_t)c->hm.m.n_atoms, ms = (size_t)(out->max_shells > 0 ? out->max_shells : 1);
+    const size_t ND = (size_t)(c->hm.m.n_dos > 1 ? c->hm.m.n_dos : 1);
+    const size_t plane = Nt * (size_t)NR;
+    size_t n = 0;
+    const double *planes[] = {out->Out_ne, out->Out_Ee, out->Out_nphot, out->Out_Ephot, out->Out_Elat};
+    for (const double *p : planes) if (p) n += plane;
+    const double *hol[] = {out->Out_nh, out->Out_Eh, out->Out_Ehkin};
+    for (const double *p : hol) if (p) n += plane * na * ms;
+    if (out->Out_E_h) n += Nt * na * ms;
+    const double *nodes[] = {out->Out_tot_Ne, out->Out_tot_Nphot, out->Out_tot_E, out->Out_E_e, out->Out_E_phot, out->Out_E_at};
+    for (const double *p : nodes) if (p) n += Nt;
+    if (want_spec) {
+        if (out->Out_Ee_vs_E) n += plane;
+        if (out->Out_Eh_vs_E) n += Nt * ND;
+        if (out->Out_theta) n += (Nt + 1) * 180;
+        if (out->Out_theta_h) n += (Nt + 1) * 180;
+    }
+    return n;
+}
+
+// The layout of a run in a region: lists of `cap` electrons and holes
+// (cap/4 photons) and B ions, both generations; one tally replica; the
+// per-iteration histograms of B iterations; the device finalize's buffers.
+static void carve_state(Carver &cv, McState &s, NoAllocBufs &na, const trk_ctx *c, int cap, int B, int NR,
+                        bool want_spec, size_t stage_n)
+{
+    const int Nt = c->hm.Nt, nsh = c->hm.nsh;
+    const int ND = c->hm.m.n_dos;
+    const int caps[4] = {cap, cap, cap / 4, B};
+    for (int g = 0; g < 2; ++g)
+        for (int k = 0; k < 4; ++k) carve_list(cv, s.L[g][k], (size_t)caps[k], k == K_HO || k == K_ION);
+    for (int k = 0; k < 4; ++k) s.cap[k] = caps[k];
+    s.d_counts = cv.take<int>(16);
+    s.d_overflow = cv.take<int>(1);
+    s.d_counters = cv.take<unsigned long long>(16);
+    const size_t cell = (size_t)Nt * NR;
+    Tally &T = s.T;
+    T.Nt = Nt; T.NR = NR; T.nsh = nsh; T.R = 1;
+    T.cnt_e = cv.take<unsigned long long>(cell);
+    T.E_e = cv.take<unsigned long long>(2 * cell);
+    T.cnt_h = cv.take<unsigned long long>(cell * nsh);
+    T.E_h = cv.take<unsigned long long>(2 * cell * nsh);
+    T.Ek_h = cv.take<unsigned long long>(2 * cell * nsh);
+    T.cnt_p = cv.take<unsigned long long>(cell);
+    T.E_p = cv.take<unsigned long long>(2 * cell);
+    T.Elat = cv.take<unsigned long long>(2 * cell);
+    s.d_tim = cv.take<double>((size_t)Nt);
+    s.d_R = cv.take<double>((size_t)NR);
+    if (want_spec) {
+        Spec &S = s.S;
+        const size_t bt = (size_t)B * Nt;
+        S.nel = cv.take<int>(bt); S.nvb = cv.take<int>(bt); S.dcn = cv.take<int>(bt);
+        S.eve = cv.take<int>(bt * NR);
+        S.ehve = cv.take<int>(bt * (ND > 1 ? ND : 1));
+        S.th = cv.take<int>(bt * 180); S.thh = cv.take<int>(bt * 180);
+        S.dcs = cv.take<unsigned long long>(bt * 2);
+        s.acc_eve = cv.take<double>((size_t)Nt * NR);
+        s.acc_ehve = cv.take<double>((size_t)Nt * (ND > 1 ? ND : 1));
+        s.acc_th = cv.take<double>((size_t)(Nt + 1) * 180);
+        s.acc_thh = cv.take<double>((size_t)(Nt + 1) * 180);
+        s.acc_dc = cv.take<double>((size_t)Nt);
+        s.spec_B = B;
+    }
+    na.stage = cv.take<double>(stage_n);
+    na.stage_n = stage_n;
+    na.d_V = cv.take<double>((size_t)NR);
+    na.node_sums = cv.take<unsigned long long>((size_t)Nt * 4);
+}
+
+// exact integer in units of 2^-40 eV -> double, as the host finalize converts
+TRK_DEV double fx_to_double_dev(__int128 v)
+{
+    long long hi = (long long)(v >> 40);
+    __int128 lo = v - ((__int128)hi << 40);
+    return (double)hi + (double)(long long)lo * 0x1.0p-40;
+}
+
+// Destinations of the device finalize (staged Out_* arrays, nullptr = skip).
+struct FinOut {
+    double *ne, *Ee, *nph, *Eph, *Elat, *nh, *Eh, *Ehk, *E_h;
+    double *tot_Ne, *tot_Nph, *tot_E, *E_e, *E_ph, *E_at;
+};
+
+// (1) one thread per (node, radial bin): the radial tallies, scaled by the
+// bin's inverse volume and added to the staged arrays (Fortran order)
+__global__ void k_fin_cells(Tally T, const double *V, FinOut o, int na, const int *atom_of, const int *shl_off)
+{
+    const int Nt = T.Nt, NR = T.NR, nsh = T.nsh;
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (int64_t)Nt * NR) return;
+    const int i = (int)(t % Nt), j = (int)(t / Nt);
+    const size_t a = (size_t)i * NR + j;
+    const double Vj = V[j];
+    const size_t oo = (size_t)t;
+    if (o.ne) o.ne[oo] += (double)(long long)T.cnt_e[a] * Vj;
+    if (o.Ee) o.Ee[oo] += fx_to_double_dev(i128_load(T.E_e + 2 * a)) * Vj;
+    if (o.nph) o.nph[oo] += (double)(long long)T.cnt_p[a] * Vj;
+    if (o.Eph) o.Eph[oo] += fx_to_double_dev(i128_load(T.E_p + 2 * a)) * Vj;
+    if (o.Elat) o.Elat[oo] += fx_to_double_dev(i128_load(T.Elat + 2 * a)) * Vj;
+    for (int s2 = 0; s2 < nsh; ++s2) {
+        const size_t q = a * nsh + s2;
+        const int at = atom_of[s2], sl = s2 - shl_off[at];
+        const size_t oh = (size_t)i + (size_t)Nt * (j + (size_t)NR * (at + (size_t)na * sl));
+        if (o.nh) o.nh[oh] += (double)(long long)T.cnt_h[q] * Vj;
+        if (o.Eh) o.Eh[oh] += fx_to_double_dev(i128_load(T.E_h + 2 * q)) * Vj;
+        if (o.Ehk) o.Ehk[oh] += fx_to_double_dev(i128_load(T.Ek_h + 2 * q)) * Vj;
+    }
+}
+
+// (2) one thread per node: totals over the radial bins and shells; the
+// lattice energy of the node and the rest of the total energy are kept for
+// the running sum of (3)
+__global__ void k_fin_nodes(Tally T, FinOut o, int na, const int *atom_of, const int *shl_off, int photons,
+                            unsigned long long *node_sums)
+{
+    const int Nt = T.Nt, NR = T.NR, nsh = T.nsh;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= Nt) return;
+    __int128 tot_e = 0, tot_Ee = 0, tot_p = 0, tot_Ep = 0, row_lat = 0, hsum = 0;
+    for (int j = 0; j < NR; ++j) {
+        const size_t a = (size_t)i * NR + j;
+        tot_e += (long long)T.cnt_e[a];
+        tot_Ee += i128_load(T.E_e + 2 * a);
+        tot_p += (long long)T.cnt_p[a];
+        tot_Ep += i128_load(T.E_p + 2 * a);
+        row_lat += i128_load(T.Elat + 2 * a);
+    }
+    for (int s2 = 0; s2 < nsh; ++s2) {
+        __int128 sh = 0;
+        for (int j = 0; j < NR; ++j) {
+            const size_t q = ((size_t)i * NR + j) * nsh + s2;
+            sh += i128_load(T.E_h + 2 * q) + i128_load(T.Ek_h + 2 * q);
+        }
+        hsum += sh;
+        const int at = atom_of[s2], sl = s2 - shl_off[at];
+        if (o.E_h) o.E_h[(size_t)i + (size_t)Nt * (at + (size_t)na * sl)] += fx_to_double_dev(sh);
+    }
+    if (o.tot_Ne) o.tot_Ne[i] += (double)(long long)tot_e;
+    if (o.E_e) o.E_e[i] += fx_to_double_dev(tot_Ee);
+    if (photons) {
+        if (o.tot_Nph) o.tot_Nph[i] += (double)(long long)tot_p;
+        if (o.E_ph) o.E_ph[i] += fx_to_double_dev(tot_Ep);
+    }
+    i128_store(node_sums + 4 * (size_t)i, row_lat);
+    i128_store(node_sums + 4 * (size_t)i + 2, tot_Ee + hsum + (photons ? tot_Ep : (__int128)0));
+}
+
+// (3) one thread: the lattice energy deposited up to each node and the total energy
+__global__ void k_fin_prefix(FinOut o, int Nt, const unsigned long long *node_sums)
+{
+    if (blockIdx.x != 0 || threadIdx.x != 0) return;
+    __int128 at_cum = 0;
+    for (int i = 0; i < Nt; ++i) {
+        at_cum += i128_load(node_sums + 4 * (size_t)i);
+        if (o.E_at) o.E_at[i] += fx_to_double_dev(at_cum);
+        if (o.tot_E) o.tot_E[i] += fx_to_double_dev(i128_load(node_sums + 4 * (size_t)i + 2) + at_cum);
+    }
+}
+
+__global__ void k_add(double *dst, const double *src, int64_t n)
+{
+    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < n) dst[k] = dst[k] + src[k];
+}
+
+static int finalize_on_device(trk_ctx *c, McState *s, const NoAllocBufs &na, const double *Out_V, int NR,
+                              trk_tallies *out, bool want_spec)
+{
+    const HostMat &h = c->hm;
+    const int Nt = h.Nt;
+    const size_t nat = (size_t)h.m.n_atoms, ms = (size_t)(out->max_shells > 0 ? out->max_shells : 1);
+    const size_t ND = (size_t)(h.m.n_dos > 1 ? h.m.n_dos : 1);
+    const size_t plane = (size_t)Nt * NR;
+    hipStream_t st = c->stream;
+    FinOut fo{};
+    struct Staged { double *host; size_t n; double **dev; };
+    Staged arr[] = {
+        {out->Out_ne, plane, &fo.ne}, {out->Out_Ee, plane, &fo.Ee}, {out->Out_nphot, plane, &fo.nph},
+        {out->Out_Ephot, plane, &fo.Eph}, {out->Out_Elat, plane, &fo.Elat},
+        {out->Out_nh, plane * nat * ms, &fo.nh}, {out->Out_Eh, plane * nat * ms, &fo.Eh},
+        {out->Out_Ehkin, plane * nat * ms, &fo.Ehk}, {out->Out_E_h, (size_t)Nt * nat * ms, &fo.E_h},
+        {out->Out_tot_Ne, (size_t)Nt, &fo.tot_Ne}, {out->Out_tot_Nphot, (size_t)Nt, &fo.tot_Nph},
+        {out->Out_tot_E, (size_t)Nt, &fo.tot_E}, {out->Out_E_e, (size_t)Nt, &fo.E_e},
+        {out->Out_E_phot, (size_t)Nt, &fo.E_ph}, {out->Out_E_at, (size_t)Nt, &fo.E_at}};
+    double *spec_dev[4] = {nullptr, nullptr, nullptr, nullptr};
+    Staged sp[] = {{want_spec ? out->Out_Ee_vs_E : nullptr, plane, &spec_dev[0]},
+                   {want_spec && h.m.n_dos > 1 ? out->Out_Eh_vs_E : nullptr, (size_t)Nt * ND, &spec_dev[1]},
+                   {want_spec ? out->Out_theta : nullptr, (size_t)(Nt + 1) * 180, &spec_dev[2]},
+                   {want_spec ? out->Out_theta_h : nullptr, (size_t)(Nt + 1) * 180, &spec_dev[3]}};
+    size_t off = 0;
+    for (Staged *group : {arr, sp}) {
+        const size_t cnt = (group == arr) ? sizeof(arr) / sizeof(arr[0]) : sizeof(sp) / sizeof(sp[0]);
+        for (size_t k = 0; k < cnt; ++k) {
+            Staged &a = group[k];
+            if (!a.host) continue;
+            if (off + a.n > na.stage_n) return set_error(c, TRK_ERR_CAPACITY, "staging area too small");
+            *a.dev = na.stage + off;
+            off += a.n;
+            HIPC(hipMemcpyAsync(*a.dev, a.host, a.n * sizeof(double), hipMemcpyHostToDevice, st));
+        }
+    }
+    HIPC(hipMemcpyAsync(na.d_V, Out_V, sizeof(double) * NR, hipMemcpyHostToDevice, st));
+    const MatDev &md = c->md;
+    k_fin_cells<<<(unsigned)((plane + 255) / 256), 256, 0, st>>>(s->T, na.d_V, fo, (int)nat, md.atom_of, md.shl_off);
+    k_fin_nodes<<<(unsigned)((Nt + 63) / 64), 64, 0, st>>>(s->T, fo, (int)nat, md.atom_of, md.shl_off,
+                                                           h.o.include_photons, na.node_sums);
+    k_fin_prefix<<<1, 64, 0, st>>>(fo, Nt, na.node_sums);
+    const double *acc[4] = {s->acc_eve, s->acc_ehve, s->acc_th, s->acc_thh};
+    for (int k = 0; k < 4; ++k)
+        if (spec_dev[k]) k_add<<<(unsigned)((sp[k].n + 255) / 256), 256, 0, st>>>(spec_dev[k], acc[k], (int64_t)sp[k].n);
+    HIPC(hipGetLastError());
+    for (Staged *group : {arr, sp}) {
+        const size_t cnt = (group == arr) ? sizeof(arr) / sizeof(arr[0]) : sizeof(sp) / sizeof(sp[0]);
+        for (size_t k = 0; k < cnt; ++k)
+            if (group[k].host)
+                HIPC(hipMemcpyAsync(group[k].host, *group[k].dev, group[k].n * sizeof(double), hipMemcpyDeviceToHost, st));
+    }
+    if (want_spec && out->Out_diff_coeff)
+        HIPC(hipMemcpyAsync(out->Out_diff_coeff, s->acc_dc, (size_t)Nt * sizeof(double), hipMemcpyDeviceToHost, st));
+    HIPC(hipStreamSynchronize(st));
+    return 0;
+}
+
+// do_Monte_Carlo on buffers carved from the scratch array: no device memory
+// is allocated or freed.  The particle-list capacity is the largest power of
+// two (2^22 down to 2^10) whose layout fits, unless the options fix it.
+int run_mc_noalloc(trk_ctx *c, int64_t nmc, uint64_t seed, int64_t first_iter, const double *Out_R,
+                   const double *Out_V, int NR, trk_tallies *out)
+{
+    if (c->trace_host && c->trace_cap > 0)
+        return set_error(c, TRK_ERR_STATE, "event traces need trk_run_mc (the run without allocation records none)");
+    int cur = -1;
+    HIPC(hipGetDevice(&cur));
+    if (cur != c->device) HIPC(hipSetDevice(c->device));
+    unsigned char *base = nullptr;
+    int rc = scratch_base(c, &base);
+    if (rc) return rc;
+    const bool want_spec = out->Out_Ee_vs_E || out->Out_Eh_vs_E || out->Out_theta || out->Out_theta_h || out->Out_diff_coeff;
+    const size_t stage_n = stage_doubles(c, out, NR, want_spec);
+    const size_t limit = (size_t)TRK_NOALLOC_SCRATCH_BYTES;
+    McState s{};
+    NoAllocBufs na{};
+    const int fixed = c->hm.o.gen_capacity;
+    int cap = 0;
+    int64_t B = 0;
+    for (int cand = (fixed > 0 ? fixed : (1 << 22)); cand >= (fixed > 0 ? fixed : (1 << 10)); cand >>= 1) {
+        const int64_t b = batch_size(c, cand, nmc);
+        Carver cv{nullptr};
+        carve_state(cv, s, na, c, cand, (int)b, NR, want_spec, stage_n);
+        if (cv.off <= limit) { cap = cand; B = b; break; }
+        if (fixed > 0) break;
+    }
+    if (cap == 0)
+        return set_error(c, TRK_ERR_CAPACITY, "the run does not fit the %zu-byte scratch array of the entry point "
+                         "without allocation (fewer time nodes, radial bins or iterations per batch would)", limit);
+    s = McState{};
+    na = NoAllocBufs{};
+    Carver cv{base};
+    carve_state(cv, s, na, c, cap, (int)B, NR, want_spec, stage_n);
+    if (device_info(c, &s)) return TRK_ERR_DEVICE;
+    rc = run_core(c, &s, nmc, seed, first_iter, Out_R, Out_V, NR, out, B, want_spec, &na);
+    if (cur != c->device) (void)hipSetDevice(cur);
+    return rc;
+}
 
 }  // namespace trk

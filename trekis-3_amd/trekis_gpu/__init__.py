@@ -109,8 +109,9 @@ class Context:
     def new_tallies(self, NR):
         return TallySet(self.Nt, NR, self.desc)
 
-    def run_mc(self, nmc, seed, Out_R, Out_V, tallies=None, first_iter=0, trace_cap=0):
-        """do_Monte_Carlo: NMC iterations added into the Out_* tallies."""
+    def run_mc(self, nmc, seed, Out_R, Out_V, tallies=None, first_iter=0, trace_cap=0, noalloc=False):
+        """do_Monte_Carlo: NMC iterations added into the Out_* tallies.  noalloc
+        runs trk_run_mc_noalloc (no device allocation; no event trace)."""
         Out_R = np.ascontiguousarray(Out_R, dtype=np.float64)
         Out_V = np.ascontiguousarray(Out_V, dtype=np.float64)
         NR = len(Out_R)
@@ -122,14 +123,14 @@ class Context:
             self.lib.trk_set_trace(self.h, dptr(trace), trace_cap, ctypes.byref(ntr))
         else:
             self.lib.trk_set_trace(self.h, None, 0, None)
-        run = self.lib.trk_run_mc
+        run = self.lib.trk_run_mc_noalloc if noalloc else self.lib.trk_run_mc
         try:
             rc = run(self.h, nmc, seed, first_iter, dptr(Out_R), dptr(Out_V), NR, ctypes.byref(t.s))
         finally:
             if trace_cap:
                 # the library must not keep a pointer into this array
                 self.lib.trk_set_trace(self.h, None, 0, None)
-        self._check(rc, "trk_run_mc")
+        self._check(rc, "trk_run_mc_noalloc" if noalloc else "trk_run_mc")
         if trace_cap:
             t.trace = trace[: min(ntr.value, trace_cap)]
             t.trace_total = ntr.value

@@ -136,6 +136,7 @@ def declare_library(lib):
         "trk_row_get": (c_int, [c_void_p, c_int, c_int, c_int32, DP, DP, c_int32]),
         "trk_time_nodes": (c_int, [c_void_p, P(c_int32)]),
         "trk_run_mc": (c_int, [c_void_p, c_int64, c_uint64, c_int64, DP, DP, c_int32, P(Tallies)]),
+        "trk_run_mc_noalloc": (c_int, [c_void_p, c_int64, c_uint64, c_int64, DP, DP, c_int32, P(Tallies)]),
         "trk_last_timing": (c_int, [c_void_p, DP, DP, P(c_int64)]),
         "trk_set_trace": (c_int, [c_void_p, DP, c_int64, P(c_int64)]),
         "trk_set_profiling": (c_int, [c_void_p, c_int]),
