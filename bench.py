@@ -175,6 +175,106 @@ def run_leg(tg, abi, args, precision, first_iter):
             "ms_shi": ms_shi, "n_shi": n_shi, "snaps": snaps, "t_tables": t_tables}
 
 
+def run_batch_leg(tg, args, precision, n_impacts):
+    """The batching workload: 4 material contexts, one ion context per impact,
+    one Batch; the autotuner picks the launch shape on the first step's
+    iterations, then --steps batches are timed (impact i of step k is
+    iteration k*n + i)."""
+    from trekis_gpu.batch import Batch, IonContext
+    lib = tg.load_library()
+    mats, opts, ions, run = tg.configs.config4_batch(n_impacts, precision)
+    t0 = time.perf_counter()
+    mctx = []
+    for m, o in zip(mats, opts):
+        m.prepare(lib)
+        c = tg.Context(m, tg.abi.Ion.from_buffer_copy(ions[0][0]), o)
+        c.build_tables()
+        mctx.append(c)
+    imps = []
+    for ion, mi in ions:
+        c = IonContext(mctx[mi], ion)
+        c.build_tables()
+        imps.append(c)
+    t_tables = time.perf_counter() - t0
+    name = "mixed" if precision else "fp64"
+    print(f"[bench] batch {name}: {len(imps)} impacts on {len(mats)} materials, tables in {t_tables:.1f} s",
+          file=sys.stderr, flush=True)
+    R, V = tg.radial_grid(run["NR"], run["Rmin"], run["Rmax"], mats[0].layer)
+    b = Batch(imps, R, V, capacity=args.batch_capacity)
+    n = len(imps)
+    tune = None
+    if args.tune:
+        tune = b.tune(run["seed"], 10 ** 6)
+        print(f"[bench] batch {name}: autotuner chose {tune}", file=sys.stderr, flush=True)
+    for w in range(args.warmup):
+        b.run(run["seed"], (args.steps + w) * n, with_tallies=False)
+    ev = np.zeros(8, dtype=np.int64)
+    t_total, dev_ms, rounds = 0.0, 0.0, []
+    tals = []
+    for k in range(args.steps):
+        ts = time.perf_counter()
+        tal = b.run(run["seed"], k * n, with_tallies=args.batch_tallies)
+        t_total += time.perf_counter() - ts
+        ms, e, r = b.timing()
+        dev_ms += ms
+        ev += e
+        rounds.append(r)
+        if tal is not None:
+            tals.append(tal)
+        print(f"[bench] batch {name}: step {k}: {int(e.sum())} events, {ms:.1f} ms on the device, {r} rounds",
+              file=sys.stderr, flush=True)
+    return {"batch": b, "ev": ev, "t_total": t_total, "dev_ms": dev_ms, "rounds": rounds, "tune": tune,
+            "tune_log": b.tune_log() if tune else None, "config": b.config(), "t_tables": t_tables, "tals": tals,
+            "mats": mats, "ions": ions, "V": V, "impacts": imps, "mctx": mctx}
+
+
+def main_batch(args):
+    import trekis_gpu as tg
+    n = args.impacts
+    mixed = args.precision == "mixed"
+    leg = run_batch_leg(tg, args, 1 if mixed else 0, n)
+    total_ev = int(leg["ev"].sum())
+    value = total_ev / leg["t_total"]
+    compare = None
+    if mixed and not args.no_compare:
+        ref = run_batch_leg(tg, args, 0, n)
+        ref_value = float(ref["ev"].sum()) / ref["t_total"]
+        compare = {"fp64_value": ref_value, "speedup_vs_fp64": value / ref_value,
+                   "fp64_autotuner_choice": ref["tune"]}
+        if leg["tals"] and ref["tals"]:
+            # radial tallies summed over each material's impacts and the steps
+            worst, per = 0.0, {}
+            for mi in range(len(leg["mats"])):
+                idx = [i for i, (_, m) in enumerate(leg["ions"]) if m == mi]
+                agg_m, agg_f = {}, {}
+                for k in set(RADIAL) | set(RADIAL.values()) | set(PHOTON) | {"Out_tot_E"}:
+                    agg_m[k] = sum(t[i][k] for t in leg["tals"] for i in idx)
+                    agg_f[k] = sum(t[i][k] for t in ref["tals"] for i in idx)
+                w, pm, de, nodes = tally_errors(agg_f, agg_m, leg["V"])
+                worst = max(worst, w)
+                per[leg["mats"][mi].name] = {"max_radial_tally_rel_error": w, "tot_E_rel_diff": de,
+                                             "tallies": {k: v for k, v in pm.items() if k in RADIAL}}
+            compare["max_radial_tally_rel_error"] = worst
+            compare["per_material"] = per
+    line = {
+        "metric": "collision events/s on 1 GPU, batched impacts, mixed vs FP64" if mixed
+                  else "collision events/s on 1 GPU, batched impacts (FP64 path)",
+        "value": value, "unit": "events/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": 1e3 * leg["t_total"] / args.steps, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32 sampling / f64 state and tallies" if mixed else "f64",
+        "data": "synthetic",
+        "config": {"workload": f"{n} ion impacts (Z 18-92, 1-20 MeV/u) on 4 synthetic targets, one persistent "
+                               f"launch per step, 0-1000 fs x 100 radial bins", "precision": args.precision,
+                   "impacts_per_step": n, "events": total_ev,
+                   "events_by_kind": dict(zip(["shi", "e_inel", "e_el", "h_inel", "h_el", "auger", "radiative",
+                                               "photon"], leg["ev"].tolist())),
+                   "device_ms_per_step": leg["dev_ms"] / args.steps, "rounds_per_launch": leg["rounds"],
+                   "launch": leg["config"], "autotuner": {"choice": leg["tune"], "timings": leg["tune_log"]},
+                   "table_build_s": leg["t_tables"], "vs_fp64": compare},
+    }
+    print(json.dumps(line), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -187,7 +287,12 @@ def main():
                     help="mixed only: skip the FP64 leg (no speed-up or tally error reported)")
     ap.add_argument("--cpu-iters", type=int, default=2,
                     help="oracle iterations per host thread for the CPU baseline (0 = skip)")
-    ap.add_argument("--config", default="config2")
+    ap.add_argument("--config", default="config2",
+                    help="config2: Al2O3-like / Au 11 MeV/u; config4: the batch of ion impacts on 4 materials")
+    ap.add_argument("--impacts", type=int, default=256, help="config4: impacts per batch")
+    ap.add_argument("--batch-capacity", type=int, default=1 << 22, help="config4: list capacity per kind and group")
+    ap.add_argument("--tune", type=int, default=1, help="config4: run the autotuner first (1) or not (0)")
+    ap.add_argument("--batch-tallies", type=int, default=1, help="config4: fetch every impact's tallies each step")
     ap.add_argument("--gen-capacity", type=int, default=1 << 27,
                     help="particles of one kind per round list (electrons, holes; photons get a quarter)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
@@ -209,6 +314,9 @@ def main():
               f"iterations is not built in this tree", file=sys.stderr)
         sys.exit(1)
 
+    if args.config == "config4":
+        main_batch(args)
+        return
     import trekis_gpu as tg
     from trekis_gpu import abi
     mixed = args.precision == "mixed"

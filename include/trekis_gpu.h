@@ -126,7 +126,13 @@ typedef struct trk_options {
     int32_t block;                      /* threads per workgroup for transport, 0 = automatic */
     int32_t tally_replicas;             /* copies of the tally arrays (contention), 0 = automatic */
     int32_t gen_capacity;               /* particle pairs per generation buffer, 0 = automatic */
-    int32_t reserved[3];
+    int32_t quadrature;                 /* CDF tables (inelastic electron/hole, ion):
+                                           0 the reference's quadrature in FP64 (replays the oracle bit for bit);
+                                           1 FP32 with compensated (Kahan) sums: 2000 geometric transfer-energy
+                                             intervals (4001 points) x 1000 Simpson intervals in ln q (2001
+                                             points), FP64 only for the cumulative normalisation; needs a
+                                             constant electron effective mass (El_eff_mass > 0) */
+    int32_t reserved[2];
 } trk_options;
 
 /* Tallies: caller-owned arrays in Fortran column-major order with the
@@ -188,6 +194,23 @@ int  trk_row_size(const trk_ctx *ctx, int kind, int shell, int32_t i_E, int32_t 
 /* Copy one cumulative row (hw [eV], lambda_cum [A]) to the host. */
 int  trk_row_get(const trk_ctx *ctx, int kind, int shell, int32_t i_E, double *hw, double *lam, int32_t n);
 
+/* Tables read from the reference's cache files instead of computed
+ * (Analytical_IMFPs.f90:259-383 reads OUTPUT_*_IMFPs*.dat through
+ * read_write_IMFP 585-630, and 663-711 / 745-790 / 1450-1510 / 1540-1600 the
+ * diff_CS/*.dat rows through read(FN_diff,*)).  After trk_build_tables, each
+ * call overwrites one shell's mean free path (L) and stopping power (dEdx)
+ * of a table kind -- E, when not NULL, must equal the built grid bit for bit
+ * -- or one cumulative row (hw, lambda_cum) of the same length as built;
+ * a NULL array is left as it is.  An ion context accepts only the ion's
+ * kinds (TRK_TAB_SHI, TRK_TAB_SHI_DIFF); set the others on its material.
+ * trk_tables_finish then re-derives the fused totals, logarithms, row flags
+ * and the ion's inverse transfer rows; trk_run_mc refuses to run between a
+ * set and the finish. */
+int  trk_table_set(trk_ctx *ctx, int kind, int shell, const double *E, const double *L, const double *dEdx,
+                   int32_t n);
+int  trk_row_set(trk_ctx *ctx, int kind, int shell, int32_t i_E, const double *hw, const double *lam, int32_t n);
+int  trk_tables_finish(trk_ctx *ctx);
+
 /* Number of snapshot time nodes of the run (time_do loop, Monte_Carlo.f90:572-665). */
 int  trk_time_nodes(const trk_ctx *ctx, int32_t *Nt_out);
 
@@ -216,6 +239,53 @@ int  trk_run_mc(trk_ctx *ctx, int64_t nmc, uint64_t seed, int64_t first_iter,
 #define TRK_NOALLOC_SCRATCH_BYTES ((size_t)128 << 20)
 int  trk_run_mc_noalloc(trk_ctx *ctx, int64_t nmc, uint64_t seed, int64_t first_iter,
                         const double *Out_R, const double *Out_V, int32_t NR, trk_tallies *out);
+
+/* --- batches of impacts ---------------------------------------------------
+ * The reference runs one ion on one target per program run, its MC
+ * iterations spread over OpenMP threads (do_Monte_Carlo's iteration loop,
+ * Monte_Carlo.f90:395-415).  A batch runs many impacts -- different ions,
+ * energies and materials, one MC iteration each -- in one persistent launch.
+ *
+ * trk_ctx_create_ion: a context for another ion on the material of a built
+ * context; it shares that context's material arrays and tables, and its
+ * trk_build_tables computes only the ion's own tables (SHI_TotIMFP,
+ * Cross_sections.f90:2452-2597, for its grid and initial energy).  The
+ * material context must outlive it.  It is an ordinary context otherwise
+ * (trk_run_mc works on it). */
+int  trk_ctx_create_ion(const trk_ctx *material, const trk_ion *ion, trk_ctx **out);
+
+typedef struct trk_batch trk_batch;
+
+/* impacts: contexts with built tables, all of one precision and device;
+ * those created on one material context (or that context itself) form a
+ * group that shares its tables (at most 8 groups).  Impact i of a run is
+ * iteration first_iter + i, so its events and tallies equal those of
+ * trk_run_mc(impacts[i], 1, seed, first_iter + i, ...).  capacity:
+ * particles of one kind a group's list holds per round (photons a quarter;
+ * 0 = 2^22).  Out_R/Out_V: the radial grid of every impact. */
+int  trk_batch_create(trk_ctx *const *impacts, int32_t n, const double *Out_R, const double *Out_V, int32_t NR,
+                      int32_t capacity, trk_batch **out);
+void trk_batch_destroy(trk_batch *b);
+int  trk_batch_error(const trk_batch *b, char *buf, size_t n);
+/* Run every impact once (the step sequence is a hipGraph captured on the
+ * first run and replayed after).  outs[i] (may be NULL) receives impact i's
+ * tallies as trk_run_mc would add them; the per-iteration normalised ones
+ * (Out_Ee_vs_E, Out_Eh_vs_E, Out_theta, Out_theta_h, Out_diff_coeff) must
+ * be NULL, and n_events is not filled (see trk_batch_timing). */
+int  trk_batch_run(trk_batch *b, uint64_t seed, int64_t first_iter, trk_tallies *const *outs);
+/* Event trace of the next runs, records as in trk_set_trace. */
+int  trk_batch_set_trace(trk_batch *b, double *buf, int64_t cap_records, int64_t *n_out);
+/* Autotuner: times the batch (seed, first_iter) with every candidate of
+ * threads per workgroup (512, 640, 768) and events a particle may take per
+ * round (256, 512, 1024), keeps the fastest and records each timing
+ * (trk_batch_tune_log).  No counterpart in the reference. */
+int  trk_batch_tune(trk_batch *b, uint64_t seed, int64_t first_iter);
+int  trk_batch_config(const trk_batch *b, int32_t *block, int32_t *round_events, int64_t *workgroups,
+                      int32_t *capacity, int32_t *groups);
+int  trk_batch_tune_log(const trk_batch *b, int32_t *block, int32_t *round_events, double *ms, int32_t cap, int32_t *n);
+/* Device time of the last run's graph (ms), its events by kind (as
+ * trk_tallies.n_events) and the transport rounds it took. */
+int  trk_batch_timing(const trk_batch *b, double *ms, int64_t *n_events, int32_t *rounds);
 
 /* Timing of the last trk_run_mc: seconds from the first ion launch to the end
  * of the last generation of each batch (HIP events on the library's stream),

@@ -22,7 +22,7 @@ pytestmark = pytest.mark.gpu
 SMALL = ["test", "test_metal", "test_sio2", "test_cdf", "test_polymer"]
 FULL = ["config1", "config2"]       # the Al-like/Xe and Al2O3-like/Au workloads at full size
 # the option branches the workloads above leave at their defaults: dispersion
-# kinds 2-4, ion charge models 1-4, a 300 K target, the plasmon integration
+# kinds 2 and 3, ion charge models 1-4, a 300 K target, the plasmon integration
 # limit and logarithmic time grids (configs.OPTION_WORKLOADS)
 OPTIONS = ["opt_plasmon_pole", "opt_ritchie_warm", "opt_sg_warm", "opt_fixed_zeff"]
 # table parity: test_sio2 has config5's tables (SiO2-like, U 20 MeV/u, phonon CDF, photons).

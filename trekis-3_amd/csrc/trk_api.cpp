@@ -60,9 +60,12 @@ static int validate(const trk_material *m, const trk_ion *ion, const trk_options
     if (ion->Kind_ion != 0) { why = "Brandt-Kitagawa ions are not built"; return 1; }
     if (!(o->Tim > 0.0) || !(o->dt > 0.0)) { why = "Tim and dt must be positive"; return 1; }
     if (o->dt_flag >= 1 && o->dt < 2.0) { why = "a logarithmic time grid needs dt >= 2 (RF:299)"; return 1; }
-    // 4 and up fall to the free-electron relation, the reference's case default (CS:388-390)
-    if (o->kind_of_DR < 1 || o->kind_of_DR > 4) { why = "kind_of_DR must be 1 to 4"; return 1; }
+    // kind 4 is the Delta-CDF model in the reference's TotIMFP (CS:955), which is not built
+    if (o->kind_of_DR < 1 || o->kind_of_DR > 3) { why = "kind_of_DR must be 1, 2 or 3 (4, Delta-CDF, is not built)"; return 1; }
     if (o->precision != 0 && o->precision != 1) { why = "precision must be 0 (FP64) or 1 (mixed)"; return 1; }
+    if (o->quadrature != 0 && o->quadrature != 1) { why = "quadrature must be 0 (reference, FP64) or 1 (FP32 compensated)"; return 1; }
+    if (o->quadrature == 1 && !(m->El_eff_mass > 0.0))
+        { why = "the FP32 table quadrature needs a constant electron effective mass (El_eff_mass > 0)"; return 1; }
     return 0;
 }
 
@@ -216,7 +219,9 @@ int trk_build_tables(trk_ctx *c)
 {
     if (!c) return TRK_ERR_ARG;
     std::string err;
-    int rc = build_tables(c->hm, c->md, c->ts, c->stream, err);
+    if (c->base && !c->base->tables_ready)
+        return set_error(c, TRK_ERR_STATE, "the material context's tables are not built");
+    int rc = build_tables(c->hm, c->md, c->ts, c->stream, err, c->base ? &c->base->ts : nullptr);
     if (rc) return set_error(c, rc, "table build failed: %s", err.c_str());
     c->tables_ready = true;
     return 0;
@@ -279,6 +284,78 @@ int trk_row_get(const trk_ctx *c, int kind, int shell, int32_t i_E, double *hw, 
     return 0;
 }
 
+// Tables replaced from host arrays: same grid, same row lengths as built.
+static int check_settable(trk_ctx *c, int kind)
+{
+    if (!c->tables_ready) return set_error(c, TRK_ERR_STATE, "trk_build_tables must run before tables are set");
+    if (kind < 0 || kind > 6) return set_error(c, TRK_ERR_ARG, "no table kind %d", kind);
+    if (c->base && kind != TRK_TAB_SHI && kind != TRK_TAB_SHI_DIFF)
+        return set_error(c, TRK_ERR_ARG, "an ion context shares its material's tables; set them on the material context");
+    return 0;
+}
+
+int trk_table_set(trk_ctx *c, int kind, int shell, const double *E, const double *L, const double *dEdx, int32_t n)
+{
+    if (!c) return TRK_ERR_ARG;
+    int rc = check_settable(c, kind);
+    if (rc) return rc;
+    TableDev &t = c->ts.tab[kind];
+    hipStream_t st = c->stream;
+    if (kind == TRK_TAB_SHI_DIFF) {
+        if (shell < 0 || shell >= c->hm.nsh || n != t.dn[shell])
+            return set_error(c, TRK_ERR_ARG, "the ion's transfer row of shell %d has %d nodes", shell,
+                             (shell >= 0 && shell < c->hm.nsh) ? t.dn[shell] : -1);
+        const int64_t o = t.doff[shell];
+        if (E && hipMemcpyAsync(t.dE + o, E, sizeof(double) * n, hipMemcpyHostToDevice, st)) return set_error(c, TRK_ERR_DEVICE, "copy");
+        if (L && hipMemcpyAsync(t.dL + o, L, sizeof(double) * n, hipMemcpyHostToDevice, st)) return set_error(c, TRK_ERR_DEVICE, "copy");
+    } else {
+        if (shell < 0 || shell >= t.nsh || n != t.n)
+            return set_error(c, TRK_ERR_ARG, "table kind %d has %d energy nodes and %d shells", kind, t.n, t.nsh);
+        if (E) {
+            // the grid is the one the tables were built on: it must match bit for bit
+            std::vector<double> g((size_t)n);
+            if (hipMemcpy(g.data(), t.E, sizeof(double) * n, hipMemcpyDeviceToHost)) return set_error(c, TRK_ERR_DEVICE, "copy");
+            for (int i = 0; i < n; ++i)
+                if (!(g[i] == E[i])) return set_error(c, TRK_ERR_ARG, "energy node %d differs from the built grid", i);
+        }
+        if (L && hipMemcpyAsync(t.L + (size_t)shell * n, L, sizeof(double) * n, hipMemcpyHostToDevice, st))
+            return set_error(c, TRK_ERR_DEVICE, "copy");
+        if (dEdx && hipMemcpyAsync(t.dEdx + (size_t)shell * n, dEdx, sizeof(double) * n, hipMemcpyHostToDevice, st))
+            return set_error(c, TRK_ERR_DEVICE, "copy");
+    }
+    if (hipStreamSynchronize(st)) return set_error(c, TRK_ERR_DEVICE, "synchronize");
+    c->tables_dirty = true;
+    return 0;
+}
+
+int trk_row_set(trk_ctx *c, int kind, int shell, int32_t i_E, const double *hw, const double *lam, int32_t n)
+{
+    if (!c) return TRK_ERR_ARG;
+    int rc = check_settable(c, kind);
+    if (rc) return rc;
+    int32_t len = 0;
+    rc = trk_row_size(c, kind, shell, i_E, &len);
+    if (rc) return set_error(c, TRK_ERR_ARG, "no cumulative row (kind %d, shell %d, node %d)", kind, shell, i_E);
+    if (n != len) return set_error(c, TRK_ERR_ARG, "the row has %d entries, not %d", len, n);
+    const TableDev &t = c->ts.tab[kind];
+    const int64_t o = t.row_off_h[(size_t)shell * t.n + i_E];
+    if (hw && hipMemcpy(t.hw + o, hw, sizeof(double) * n, hipMemcpyHostToDevice)) return set_error(c, TRK_ERR_DEVICE, "copy");
+    if (lam && hipMemcpy(t.lam + o, lam, sizeof(double) * n, hipMemcpyHostToDevice)) return set_error(c, TRK_ERR_DEVICE, "copy");
+    c->tables_dirty = true;
+    return 0;
+}
+
+int trk_tables_finish(trk_ctx *c)
+{
+    if (!c) return TRK_ERR_ARG;
+    if (!c->tables_ready) return set_error(c, TRK_ERR_STATE, "trk_build_tables must run first");
+    std::string err;
+    const int rc = refresh_derived(c->hm, c->ts, c->stream, err);
+    if (rc) return set_error(c, rc, "deriving tables failed: %s", err.c_str());
+    c->tables_dirty = false;
+    return 0;
+}
+
 int trk_time_nodes(const trk_ctx *c, int32_t *Nt_out)
 {
     if (!c || !Nt_out) return TRK_ERR_ARG;
@@ -291,6 +368,8 @@ static int check_run_args(trk_ctx *c, int64_t nmc, int64_t first_iter, const dou
 {
     if (!c) return TRK_ERR_ARG;
     if (!c->tables_ready) return set_error(c, TRK_ERR_STATE, "trk_build_tables must run before trk_run_mc");
+    if (c->tables_dirty || (c->base && c->base->tables_dirty))
+        return set_error(c, TRK_ERR_STATE, "tables were set: trk_tables_finish must run before trk_run_mc");
     if (!out || !Out_R || !Out_V || NR < 1 || nmc < 1 || first_iter < 0 || first_iter + nmc > 0xFFFFFFFFll)
         return set_error(c, TRK_ERR_ARG, "invalid trk_run_mc arguments");
     if (out->Nt != c->hm.Nt) return set_error(c, TRK_ERR_ARG, "tallies have Nt=%d, the run has %d time nodes", out->Nt, c->hm.Nt);

@@ -47,10 +47,14 @@ struct trk_ctx {
     trk::MatDev md;
     trk::TableSet ts;
     bool tables_ready = false;
+    bool tables_dirty = false;     // tables set from host arrays, not yet finished
     hipStream_t stream = nullptr;
     int device = 0;
     std::string err;
     trk::McState *mc = nullptr;
+    // an ion context (trk_ctx_create_ion): the material context whose
+    // material arrays and tables this one uses (it must outlive this one)
+    const trk_ctx *base = nullptr;
     // timing of the last run
     double last_transport_s = 0, last_total_s = 0;
     int64_t last_launches = 0;

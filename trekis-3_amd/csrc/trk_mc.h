@@ -40,6 +40,41 @@ struct Spec {
 // tables staged in each transport workgroup's LDS (see stage_tables)
 constexpr int kStaged = 26;
 
+// The ion of one impact: its parameters and its own tables (SHI_MFP, the
+// fused SHI_path, diff_SHI_MFP at its initial energy and the mixed path's
+// per-shell transfer constants).  A context has one; a batch keeps one per
+// impact in device memory, read by the lanes that run an ion's collision.
+struct IonDev {
+    double mass, E0, Zeff0, fixed_zeff;   // [proton masses], [eV], charge at E0, fixed charge
+    int Z, kind_zeff, nS, pad_;
+    const double *ES, *LSsh, *shi_path, *lnES, *ln_LSsh, *ln_shi_path;
+    const int *sd_n; const int64_t *sd_off; const double *sd_E, *sd_L, *sd_inv; const int *sd_nonmono;
+    const int *sd_M; const double *sd_c;
+};
+
+// Round state of one material group of a batch (device memory): particles
+// of the current round by kind, the next round's allocation counters by list
+// parity, the shared cursor and the parity of the current round's lists.
+struct GroupCtl {
+    int n_in[4];
+    int out_cnt[2][4];
+    int cursor;
+    int cur;
+    int overflow;
+    int pad_;
+};
+
+// Control of a batch launch (device memory): the grid barrier, the round
+// number, the end flag, and the error bits (barrier time-out).
+struct BatchCtl {
+    unsigned bar_count, bar_gen;
+    int done, err;
+    int round, n_groups;
+    unsigned long long deadline_ticks;   // wall-clock ticks a barrier may wait
+};
+constexpr int kBatchErrBarrier = 1;
+constexpr int kMaxGroups = 8;
+
 struct McDev {
     MatDev m;
     // tables
@@ -85,6 +120,13 @@ struct McDev {
     // its per-shell paths and of the fused path
     const int *sd_M; const double *sd_c;
     const double *lnES, *ln_LSsh, *ln_shi_path;
+    // batch runs only: the impacts' ions (indexed by iteration - iter0), the
+    // events a particle may take per round, this group's round state
+    const IonDev *ions;
+    const int *plane_of;        // tally plane of each impact in its group
+    uint32_t iter0;
+    int round_events;
+    GroupCtl *gctl;
 };
 
 enum { EV_SHI = 0, EV_E_INEL, EV_E_EL, EV_H_INEL, EV_H_EL, EV_AUGER, EV_RAD, EV_PHOT };
@@ -126,9 +168,39 @@ constexpr int kTransportBlockMx = 768;
 #define TRK_MC_KERNELS                                                          \
     template <bool kAll> __global__ void k_transport(McDev din);             \
     __global__ void k_ion_init(McDev d, int nb, uint32_t first_iter);        \
-    __global__ void k_shi_consts(McDev d, int nsh, int *sd_M, double *sd_c);
+    __global__ void k_shi_consts(IonDev iv, const double *Ip, int nsh, int *sd_M, double *sd_c); \
+    template <bool kAll> __global__ void k_batch(BatchCtl *ctl, int n_groups); \
+    __global__ void k_ion_init_batch(int g, int nb, const int *idx);        \
+    hipError_t batch_upload(const McDev *mc, int n_groups, hipStream_t st);
 namespace fp64 { TRK_MC_KERNELS }
 namespace mx { TRK_MC_KERNELS }
 #undef TRK_MC_KERNELS
+
+// The ion view of a context's own ion (host side).
+IonDev ion_view_host(const trk_ctx *c);
+// Transport descriptor of a context's tables (trk_mc.hip).
+void fill_mcdev(const trk_ctx *c, McDev &d);
+// LDS layout of the staged tables (trk_mc.hip); true when all are staged.
+bool plan_lds(McDev &d, int Nt, int NR, int n_dos, int nsh, size_t limit_doubles, size_t *bytes);
+// One tally plane converted and added into the caller's Out_* arrays (trk_mc.hip).
+int finish_tallies_host(trk_ctx *c, const Tally &T, int plane_index, const double *Out_V, int NR, trk_tallies *out,
+                        hipStream_t st);
+
+// 128-bit tally words: (low, high) 64-bit halves of an exact integer
+TRK_DEV __int128 fx_read(const unsigned long long *p)
+{
+    return (__int128)(long long)p[1] * ((__int128)1 << 32) + (__int128)p[0];
+}
+
+TRK_DEV void i128_store(unsigned long long *p, __int128 v)
+{
+    p[0] = (unsigned long long)v;
+    p[1] = (unsigned long long)(v >> 64);
+}
+
+TRK_DEV __int128 i128_load(const unsigned long long *p)
+{
+    return (__int128)(((unsigned __int128)p[1] << 64) | (unsigned __int128)p[0]);
+}
 
 }  // namespace trk

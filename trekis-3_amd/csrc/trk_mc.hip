@@ -139,17 +139,6 @@ __global__ void k_diff_coeff(double *D, const unsigned long long *dcs, const int
 // units of 2^-40 eV.  Two kernels: the replica sum per cell, then the running
 // sum per column (electrons and photons by radial bin, holes by radial bin
 // and shell; the lattice energy per node has no prefix).
-TRK_DEV __int128 fx_read(const unsigned long long *p)
-{
-    return (__int128)(long long)p[1] * ((__int128)1 << 32) + (__int128)p[0];
-}
-
-TRK_DEV void i128_store(unsigned long long *p, __int128 v)
-{
-    p[0] = (unsigned long long)v;
-    p[1] = (unsigned long long)(v >> 64);
-}
-
 // (1) one thread per cell: the replicas summed into replica 0 (counts as
 // int64, energies as int128)
 __global__ void k_replica_sum(Tally T)
@@ -187,11 +176,6 @@ __global__ void k_replica_sum(Tally T)
         i128_store(T.E_h + 2 * a, sE);
         i128_store(T.Ek_h + 2 * a, sK);
     }
-}
-
-TRK_DEV __int128 i128_load(const unsigned long long *p)
-{
-    return (__int128)(((unsigned __int128)p[1] << 64) | (unsigned __int128)p[0]);
 }
 
 // (2) one thread per column: the running sum over the time nodes of the
@@ -382,7 +366,7 @@ static int device_info(trk_ctx *c, McState *s)
     return 0;
 }
 
-static void fill_mcdev(const trk_ctx *c, McDev &d)
+void fill_mcdev(const trk_ctx *c, McDev &d)
 {
     const TableSet &ts = c->ts;
     const TableDev *T = ts.tab;
@@ -427,6 +411,77 @@ static double fx_to_double(__int128 v)
     return (double)hi + (double)(long long)lo * 0x1.0p-40;
 }
 
+// The tallies of plane `plane` of T (replicas summed and time prefix taken
+// on the device, exact integers): converted, scaled by Out_V and added into
+// the caller's Out_* arrays, with the totals per time node formed here.
+int finish_tallies_host(trk_ctx *c, const Tally &T, int plane_index, const double *Out_V, int NR, trk_tallies *out,
+                        hipStream_t st)
+{
+    const HostMat &h = c->hm;
+    const int Nt = h.Nt, nsh = h.nsh;
+    const size_t plane = (size_t)Nt * NR;
+    const size_t p0 = (size_t)plane_index * plane;
+    std::vector<unsigned long long> he(plane), hEe(2 * plane), hh(plane * nsh), hEh(2 * plane * nsh),
+        hEk(2 * plane * nsh), hp(plane), hEp(2 * plane), hEl(2 * plane);
+    HIPC(hipMemcpyAsync(he.data(), T.cnt_e + p0, plane * 8, hipMemcpyDeviceToHost, st));
+    HIPC(hipMemcpyAsync(hEe.data(), T.E_e + 2 * p0, plane * 16, hipMemcpyDeviceToHost, st));
+    HIPC(hipMemcpyAsync(hh.data(), T.cnt_h + p0 * nsh, plane * nsh * 8, hipMemcpyDeviceToHost, st));
+    HIPC(hipMemcpyAsync(hEh.data(), T.E_h + 2 * p0 * nsh, plane * nsh * 16, hipMemcpyDeviceToHost, st));
+    HIPC(hipMemcpyAsync(hEk.data(), T.Ek_h + 2 * p0 * nsh, plane * nsh * 16, hipMemcpyDeviceToHost, st));
+    HIPC(hipMemcpyAsync(hp.data(), T.cnt_p + p0, plane * 8, hipMemcpyDeviceToHost, st));
+    HIPC(hipMemcpyAsync(hEp.data(), T.E_p + 2 * p0, plane * 16, hipMemcpyDeviceToHost, st));
+    HIPC(hipMemcpyAsync(hEl.data(), T.Elat + 2 * p0, plane * 16, hipMemcpyDeviceToHost, st));
+    HIPC(hipStreamSynchronize(st));
+    auto i128 = [](const unsigned long long *p) { return (__int128)(((unsigned __int128)p[1] << 64) | p[0]); };
+    const int na = h.m.n_atoms;
+    __int128 at_cum = 0;
+    std::vector<__int128> shE(nsh);
+    for (int i = 0; i < Nt; ++i) {
+        __int128 tot_e = 0, tot_Ee = 0, tot_p = 0, tot_Ep = 0, row_lat = 0;
+        for (int s2 = 0; s2 < nsh; ++s2) shE[s2] = 0;
+        for (int j = 0; j < NR; ++j) {
+            const size_t a = (size_t)i * NR + j;
+            const long long ne_ = (long long)he[a], np_ = (long long)hp[a];
+            const __int128 Ee = i128(&hEe[2 * a]), Ep = i128(&hEp[2 * a]), El = i128(&hEl[2 * a]);
+            tot_e += ne_; tot_Ee += Ee; tot_p += np_; tot_Ep += Ep; row_lat += El;
+            const double V = Out_V[j];
+            const size_t o = (size_t)i + (size_t)Nt * j;
+            if (out->Out_ne) out->Out_ne[o] += (double)ne_ * V;
+            if (out->Out_Ee) out->Out_Ee[o] += fx_to_double(Ee) * V;
+            if (out->Out_nphot) out->Out_nphot[o] += (double)np_ * V;
+            if (out->Out_Ephot) out->Out_Ephot[o] += fx_to_double(Ep) * V;
+            if (out->Out_Elat) out->Out_Elat[o] += fx_to_double(El) * V;
+            for (int s2 = 0; s2 < nsh; ++s2) {
+                const size_t q = a * nsh + s2;
+                const long long nh_ = (long long)hh[q];
+                const __int128 Eh = i128(&hEh[2 * q]), Ek = i128(&hEk[2 * q]);
+                shE[s2] += Eh + Ek;
+                int at = c->hm.atom_of[s2], sl = s2 - c->hm.shl_off[at];
+                size_t oh = (size_t)i + (size_t)Nt * (j + (size_t)NR * (at + (size_t)na * sl));
+                if (out->Out_nh) out->Out_nh[oh] += (double)nh_ * V;
+                if (out->Out_Eh) out->Out_Eh[oh] += fx_to_double(Eh) * V;
+                if (out->Out_Ehkin) out->Out_Ehkin[oh] += fx_to_double(Ek) * V;
+            }
+        }
+        at_cum += row_lat;
+        __int128 hsum = 0;
+        for (int s2 = 0; s2 < nsh; ++s2) {
+            hsum += shE[s2];
+            int at = c->hm.atom_of[s2], sl = s2 - c->hm.shl_off[at];
+            if (out->Out_E_h) out->Out_E_h[(size_t)i + (size_t)Nt * (at + (size_t)na * sl)] += fx_to_double(shE[s2]);
+        }
+        if (out->Out_tot_Ne) out->Out_tot_Ne[i] += (double)(long long)tot_e;
+        if (out->Out_E_e) out->Out_E_e[i] += fx_to_double(tot_Ee);
+        if (out->Out_E_at) out->Out_E_at[i] += fx_to_double(at_cum);
+        if (h.o.include_photons) {
+            if (out->Out_tot_Nphot) out->Out_tot_Nphot[i] += (double)(long long)tot_p;
+            if (out->Out_E_phot) out->Out_E_phot[i] += fx_to_double(tot_Ep);
+        }
+        if (out->Out_tot_E) out->Out_tot_E[i] += fx_to_double(tot_Ee + hsum + (h.o.include_photons ? tot_Ep : 0) + at_cum);
+    }
+    return 0;
+}
+
 // Each generation is the set of particles born in the previous one; a chain of
 // ionisations is at most a few hundred long.
 constexpr int kMaxGenerations = 1 << 16;
@@ -441,6 +496,49 @@ static int64_t batch_size(const trk_ctx *c, int cap, int64_t nmc)
     if (B < 1) B = 1;
     if (B > nmc) B = nmc;
     return B;
+}
+
+// LDS layout of the tables a transport workgroup stages (in this order while
+// they fit in `limit` doubles): sets d.lds_n / d.lds_off, the bytes in
+// *bytes, and returns whether every array is staged.
+bool plan_lds(McDev &d, int Nt, int NR, int n_dos, int nsh, size_t limit, size_t *bytes)
+{
+    const int nsz[kStaged] = {d.nE, d.nE, d.nE, d.nE, d.nEel, d.nEel, d.nEel, d.nEel,
+                              d.nH, d.nH, d.nH, d.nH, d.nHel, d.nHel, d.nHel, d.nHel,
+                              NR, Nt, n_dos, n_dos, n_dos, n_dos, nsh * d.nE, nsh * d.nH,
+                              nsh * d.nE, nsh * d.nH};
+    const bool present[kStaged] = {true, true, d.lnE != nullptr, d.ln_el_imfp != nullptr, true, true,
+                                   d.lnEel != nullptr, d.ln_Lel != nullptr, true, true, d.lnEH != nullptr,
+                                   d.ln_hole_imfp != nullptr, true, true, d.lnEHel != nullptr,
+                                   d.ln_LHel != nullptr, true, true, true, true, true, true, true, true,
+                                   d.ln_Lsh != nullptr, d.ln_LHsh != nullptr};
+    size_t off = 0;
+    bool all = true;
+    for (int a = 0; a < kStaged; ++a) {
+        size_t n = present[a] ? (size_t)nsz[a] : 0;
+        if (off + n > limit) n = 0;
+        if (!present[a] || n != (size_t)nsz[a] || n == 0) all = false;
+        d.lds_n[a] = (int)n;
+        d.lds_off[a] = (int)off;
+        off += n;
+    }
+    *bytes = off * sizeof(double);
+    return all;
+}
+
+// The ion of a context as the event code reads it.
+IonDev ion_view_host(const trk_ctx *c)
+{
+    const TableSet &ts = c->ts;
+    const TableDev &td = ts.tab[TRK_TAB_SHI_DIFF];
+    IonDev v{};
+    v.mass = c->md.ion_mass; v.E0 = c->md.ion_E; v.Zeff0 = c->md.ion_Zeff0; v.fixed_zeff = c->md.ion_fixed_zeff;
+    v.Z = c->md.ion_Z; v.kind_zeff = c->md.ion_kind_zeff; v.nS = ts.tab[TRK_TAB_SHI].n;
+    v.ES = ts.tab[TRK_TAB_SHI].E; v.LSsh = ts.tab[TRK_TAB_SHI].L; v.shi_path = ts.shi_path;
+    v.lnES = ts.ln_E[TRK_TAB_SHI]; v.ln_LSsh = ts.ln_L[TRK_TAB_SHI]; v.ln_shi_path = ts.ln_shi_path;
+    v.sd_n = ts.sd_n; v.sd_off = ts.sd_off; v.sd_E = td.dE; v.sd_L = td.dL; v.sd_inv = td.dInv;
+    v.sd_nonmono = td.dNonMono; v.sd_M = ts.sd_M; v.sd_c = ts.sd_c;
+    return v;
 }
 
 // Buffers of the device finalize used by runs without allocation: the
@@ -490,7 +588,7 @@ static int run_core(trk_ctx *c, McState *s, int64_t nmc, uint64_t seed, int64_t 
     // per table set with its own arithmetic (the FP64 path evaluates them in
     // every event)
     if (mixed && !c->ts.sd_consts) {
-        mx::k_shi_consts<<<(nsh + 63) / 64, 64, 0, st>>>(d, nsh, c->ts.sd_M, c->ts.sd_c);
+        mx::k_shi_consts<<<(nsh + 63) / 64, 64, 0, st>>>(ion_view_host(c), c->md.Ip, nsh, c->ts.sd_M, c->ts.sd_c);
         HIPC(hipGetLastError());
         c->ts.sd_consts = true;
     }
@@ -542,32 +640,13 @@ static int run_core(trk_ctx *c, McState *s, int64_t nmc, uint64_t seed, int64_t 
 
     // LDS plan: stage the arrays in order while they fit
     {
-        const int nsz[kStaged] = {d.nE, d.nE, d.nE, d.nE, d.nEel, d.nEel, d.nEel, d.nEel,
-                                  d.nH, d.nH, d.nH, d.nH, d.nHel, d.nHel, d.nHel, d.nHel,
-                                  NR, Nt, h.m.n_dos, h.m.n_dos, h.m.n_dos, h.m.n_dos, nsh * d.nE, nsh * d.nH,
-                                  nsh * d.nE, nsh * d.nH};
-        const bool present[kStaged] = {true, true, d.lnE != nullptr, d.ln_el_imfp != nullptr, true, true,
-                                       d.lnEel != nullptr, d.ln_Lel != nullptr, true, true, d.lnEH != nullptr,
-                                       d.ln_hole_imfp != nullptr, true, true, d.lnEHel != nullptr,
-                                       d.ln_LHel != nullptr, true, true, true, true, true, true, true, true,
-                                       d.ln_Lsh != nullptr, d.ln_LHsh != nullptr};
         size_t limit = (size_t)s->max_lds / sizeof(double);
         // TRK_LDS_LIMIT_BYTES lowers the budget (tests of the partly staged kernel)
         if (const char *e = getenv("TRK_LDS_LIMIT_BYTES")) {
             const long long v = atoll(e);
             if (v >= 0 && (size_t)v / sizeof(double) < limit) limit = (size_t)v / sizeof(double);
         }
-        size_t off = 0;
-        bool all = true;
-        for (int a = 0; a < kStaged; ++a) {
-            size_t n = present[a] ? (size_t)nsz[a] : 0;
-            if (off + n > limit) n = 0;
-            if (!present[a] || n != (size_t)nsz[a] || n == 0) all = false;
-            d.lds_n[a] = (int)n;
-            d.lds_off[a] = (int)off;
-            off += n;
-        }
-        s->lds_bytes = off * sizeof(double);
+        const bool all = plan_lds(d, Nt, NR, h.m.n_dos, nsh, limit, &s->lds_bytes);
         s->all_staged = all;
         const void *kt = mixed ? (all ? (const void *)mx::k_transport<true> : (const void *)mx::k_transport<false>)
                                : (all ? (const void *)fp64::k_transport<true> : (const void *)fp64::k_transport<false>);
@@ -731,64 +810,9 @@ static int run_core(trk_ctx *c, McState *s, int64_t nmc, uint64_t seed, int64_t 
         c->last_launches = launches;
         return 0;
     }
-    const size_t plane = (size_t)Nt * NR;
-    std::vector<unsigned long long> he(plane), hEe(2 * plane), hh(plane * nsh), hEh(2 * plane * nsh),
-        hEk(2 * plane * nsh), hp(plane), hEp(2 * plane), hEl(2 * plane);
-    HIPC(hipMemcpyAsync(he.data(), T.cnt_e, plane * 8, hipMemcpyDeviceToHost, st));
-    HIPC(hipMemcpyAsync(hEe.data(), T.E_e, plane * 16, hipMemcpyDeviceToHost, st));
-    HIPC(hipMemcpyAsync(hh.data(), T.cnt_h, plane * nsh * 8, hipMemcpyDeviceToHost, st));
-    HIPC(hipMemcpyAsync(hEh.data(), T.E_h, plane * nsh * 16, hipMemcpyDeviceToHost, st));
-    HIPC(hipMemcpyAsync(hEk.data(), T.Ek_h, plane * nsh * 16, hipMemcpyDeviceToHost, st));
-    HIPC(hipMemcpyAsync(hp.data(), T.cnt_p, plane * 8, hipMemcpyDeviceToHost, st));
-    HIPC(hipMemcpyAsync(hEp.data(), T.E_p, plane * 16, hipMemcpyDeviceToHost, st));
-    HIPC(hipMemcpyAsync(hEl.data(), T.Elat, plane * 16, hipMemcpyDeviceToHost, st));
-    HIPC(hipStreamSynchronize(st));
-    auto i128 = [](const unsigned long long *p) { return (__int128)(((unsigned __int128)p[1] << 64) | p[0]); };
-    const int na = h.m.n_atoms;
-    __int128 at_cum = 0;
-    std::vector<__int128> shE(nsh);
-    for (int i = 0; i < Nt; ++i) {
-        __int128 tot_e = 0, tot_Ee = 0, tot_p = 0, tot_Ep = 0, row_lat = 0;
-        for (int s2 = 0; s2 < nsh; ++s2) shE[s2] = 0;
-        for (int j = 0; j < NR; ++j) {
-            const size_t a = (size_t)i * NR + j;
-            const long long ne_ = (long long)he[a], np_ = (long long)hp[a];
-            const __int128 Ee = i128(&hEe[2 * a]), Ep = i128(&hEp[2 * a]), El = i128(&hEl[2 * a]);
-            tot_e += ne_; tot_Ee += Ee; tot_p += np_; tot_Ep += Ep; row_lat += El;
-            const double V = Out_V[j];
-            const size_t o = (size_t)i + (size_t)Nt * j;
-            if (out->Out_ne) out->Out_ne[o] += (double)ne_ * V;
-            if (out->Out_Ee) out->Out_Ee[o] += fx_to_double(Ee) * V;
-            if (out->Out_nphot) out->Out_nphot[o] += (double)np_ * V;
-            if (out->Out_Ephot) out->Out_Ephot[o] += fx_to_double(Ep) * V;
-            if (out->Out_Elat) out->Out_Elat[o] += fx_to_double(El) * V;
-            for (int s2 = 0; s2 < nsh; ++s2) {
-                const size_t q = a * nsh + s2;
-                const long long nh_ = (long long)hh[q];
-                const __int128 Eh = i128(&hEh[2 * q]), Ek = i128(&hEk[2 * q]);
-                shE[s2] += Eh + Ek;
-                int at = c->hm.atom_of[s2], sl = s2 - c->hm.shl_off[at];
-                size_t oh = (size_t)i + (size_t)Nt * (j + (size_t)NR * (at + (size_t)na * sl));
-                if (out->Out_nh) out->Out_nh[oh] += (double)nh_ * V;
-                if (out->Out_Eh) out->Out_Eh[oh] += fx_to_double(Eh) * V;
-                if (out->Out_Ehkin) out->Out_Ehkin[oh] += fx_to_double(Ek) * V;
-            }
-        }
-        at_cum += row_lat;
-        __int128 hsum = 0;
-        for (int s2 = 0; s2 < nsh; ++s2) {
-            hsum += shE[s2];
-            int at = c->hm.atom_of[s2], sl = s2 - c->hm.shl_off[at];
-            if (out->Out_E_h) out->Out_E_h[(size_t)i + (size_t)Nt * (at + (size_t)na * sl)] += fx_to_double(shE[s2]);
-        }
-        if (out->Out_tot_Ne) out->Out_tot_Ne[i] += (double)(long long)tot_e;
-        if (out->Out_E_e) out->Out_E_e[i] += fx_to_double(tot_Ee);
-        if (out->Out_E_at) out->Out_E_at[i] += fx_to_double(at_cum);
-        if (h.o.include_photons) {
-            if (out->Out_tot_Nphot) out->Out_tot_Nphot[i] += (double)(long long)tot_p;
-            if (out->Out_E_phot) out->Out_E_phot[i] += fx_to_double(tot_Ep);
-        }
-        if (out->Out_tot_E) out->Out_tot_E[i] += fx_to_double(tot_Ee + hsum + (h.o.include_photons ? tot_Ep : 0) + at_cum);
+    {
+        const int frc = finish_tallies_host(c, T, 0, Out_V, NR, out, st);
+        if (frc) return frc;
     }
     if (want_spec) {
         std::vector<double> buf;

@@ -9,5 +9,7 @@ namespace mx {
 
 template __global__ void k_transport<true>(McDev din);
 template __global__ void k_transport<false>(McDev din);
+template __global__ void k_batch<true>(BatchCtl *ctl, int n_groups);
+template __global__ void k_batch<false>(BatchCtl *ctl, int n_groups);
 }  // namespace mx
 }  // namespace trk

@@ -97,5 +97,6 @@ struct HostMat;  // defined in trk_ctx.h
 int build_tables(const HostMat &hm, const MatDev &md, TableSet &ts, hipStream_t st, std::string &err,
                  const TableSet *mat = nullptr);
 void free_tables(TableSet &ts);
+int refresh_derived(const HostMat &hm, TableSet &ts, hipStream_t st, std::string &err);
 
 }  // namespace trk

@@ -193,7 +193,94 @@ struct RowJob {
     double Ele, Estart, Emax, E_low, E_high, dE_min, Mass, Zeff, MSHI;
     int n, shell, type, with_row;
     int kind, ie;
+    int q32;     // FP32 compensated quadrature (ROW_INEL, ROW_SHI, ROW_SHI_DIFF)
+    int pad_;
 };
+
+// ---------------------------------------------------------------- FP32 quadrature
+// The same cross sections with the quadrature of the precision form: kQ32W
+// transfer-energy intervals, geometric from Estart to Emax, each with its
+// midpoint (4001 points), and at each point kQ32Q Simpson intervals in ln q
+// (2001 points), evaluated and summed in FP32 with compensated (Kahan) sums.
+// Momenta are carried as q/q1 with q1 = sqrt(2 m_e * 1 eV)/hbar, so that
+// hbar^2 q^2 / (2 m* m_e) is (q/q1)^2 / m* eV and every FP32 value stays in
+// range; int dq/q is int d(ln q), unchanged by the scale.
+constexpr int kQ32W = 2000;
+constexpr int kQ32Q = 1000;
+
+struct Kahan32 {
+    float s = 0.0f, c = 0.0f;
+    TRK_DEV void add(float x)
+    {
+        const float y = x - c;
+        const float t = s + y;
+        c = (t - s) - y;
+        s = t;
+    }
+    TRK_DEV double value() const { return (double)s - (double)c; }
+};
+
+// Imewq with Loss_func and the dispersion of Extend_E0_to_finite_q (CS:346-486)
+// for a constant effective mass, in FP32; q2 = (q/q1)^2.
+TRK_DEV float imewq32(const MatDev &m, int shell, float w, float q2, float inv_mstar, float c_vf)
+{
+    const float sqq = q2 * inv_mstar;
+    const float w2 = w * w;
+    float ImE = 0.0f;
+    const int o1 = m.osc_off[shell + 1];
+    for (int i = m.osc_off[shell]; i < o1; ++i) {
+        const float E = (float)m.oE0[i], G = (float)m.oG[i], A = (float)m.oA[i];
+        float E0, G1;
+        if (m.kind_of_DR == 2) {
+            E0 = sqrtf(E * E + c_vf * q2 + sqq * sqq);
+            G1 = G;
+        } else if (m.kind_of_DR == 3) {
+            E0 = powf(powf(E, 0.666666666666f) + powf(sqq, 0.666666666666f), 1.5f);
+            G1 = sqrtf(G * G + sqq * sqq);
+        } else {
+            E0 = E + sqq;
+            G1 = G;
+        }
+        const float d = w2 - E0 * E0;
+        ImE = ImE + (A * G1 * w) / (d * d + G1 * G1 * w2);
+    }
+    return ImE;
+}
+
+// One point of a row: the q integral of Diff_cross_section (CS:2217-2282) or
+// SHI_Diff_cross_section (CS:2683-2744) at transfer energy x, in FP32.
+TRK_DEV double eval32(const MatDev &m, const RowJob &j, double x)
+{
+    // limits in units of q1 (FP64), then the integral in FP32
+    double qlo, qhi, pref;
+    if (j.type == ROW_INEL) {
+        if (!(x < j.Ele)) return 0.0;
+        const double sm = trk_sqrt(j.Mass), se = trk_sqrt(j.Ele), sr = trk_sqrt(j.Ele - x);
+        qlo = sm * (se - sr);
+        qhi = sm * (se + sr);
+        pref = 1.0 / ((kPi * kA0) * j.Ele);
+    } else {
+        if (!(j.Ele > 0.0) || !(j.Emax > 0.0)) return 0.0;
+        const double q1 = trk_sqrt(2.0 * kMe) / kH;
+        qlo = ((x / kH) / trk_sqrt((2.0 * j.Ele) / j.MSHI)) / q1;
+        qhi = trk_sqrt(j.Emax);
+        pref = 1.0;
+    }
+    if (m.temp > 0.0) pref = pref / (1.0 - trk_exp(-((x / m.temp) * kKb)));
+    if (!(qlo > 0.0) || !(qhi > qlo)) return 0.0;
+    const float inv_mstar = (float)(1.0 / m.El_eff_mass);
+    const float c_vf = (float)(((m.v_f * m.v_f) * (2.0 * kMe)) / 3.0);
+    const float u0 = (float)trk_log(qlo), h = (float)((trk_log(qhi) - trk_log(qlo)) / (double)kQ32Q);
+    const float w = (float)x;
+    Kahan32 S;
+    for (int k = 0; k <= kQ32Q; ++k) {
+        const float q = __expf(u0 + h * (float)k);
+        const float f = imewq32(m, j.shell, w, q * q, inv_mstar, c_vf);
+        const float wk = (k == 0 || k == kQ32Q) ? 1.0f : ((k & 1) ? 4.0f : 2.0f);
+        S.add(wk * f);
+    }
+    return (double)((float)pref * ((h / 3.0f) * (float)S.value()));
+}
 
 __global__ void k_row_plan(const RowJob *__restrict__ jobs, int nrows, int *__restrict__ cnt, int *__restrict__ len)
 {
@@ -202,6 +289,15 @@ __global__ void k_row_plan(const RowJob *__restrict__ jobs, int nrows, int *__re
     RowJob j = jobs[r];
     int c = 0;
     double E = j.Estart;
+    if (j.q32) {
+        // the fixed geometric intervals of the FP32 quadrature (none when
+        // the range is empty); rows hold at least 10 entries, as the
+        // reference's allocation does
+        c = (j.Emax > j.Estart && j.Estart > 0.0) ? kQ32W : 0;
+        cnt[r] = c;
+        len[r] = j.with_row ? ((j.type == ROW_SHI_DIFF) ? c : (c > 10 ? c : 10)) : 0;
+        return;
+    }
     if (j.type == ROW_PHONON) {
         while (trk_fabs(E) <= trk_fabs(j.Emax) && c < kMaxRowSteps) {
             c++;
@@ -244,6 +340,18 @@ __global__ void k_row_nodes(const RowJob *__restrict__ jobs, int nrows, const in
     int64_t o = step_off[r];
     int64_t o1 = step_off[r + 1];
     double E = j.Estart;
+    if (j.q32) {
+        const double lr = trk_log(j.Emax / j.Estart) / (double)kQ32W;
+        for (int64_t k = o; k < o1; ++k) {
+            const int64_t i = k - o;
+            const double a = j.Estart * trk_exp(lr * (double)i);
+            const double b = (i + 1 == o1 - o) ? j.Emax : j.Estart * trk_exp(lr * (double)(i + 1));
+            Ek[k] = a;
+            dEk[k] = b - a;
+            step_row[k] = r;
+        }
+        return;
+    }
     for (int64_t k = o; k < o1; ++k) {
         double dE = define_dE(j.n, E, j.E_low, j.E_high, j.dE_min);
         if (j.type == ROW_PHONON) dE = 1.0 * dE;
@@ -256,6 +364,7 @@ __global__ void k_row_nodes(const RowJob *__restrict__ jobs, int nrows, const in
 
 TRK_DEV double eval_point(const MatDev &m, const RowJob &j, double x)
 {
+    if (j.q32) return eval32(m, j, x);
     switch (j.type) {
     case ROW_INEL: return diff_cs(m, j.shell, j.Ele, x, j.Mass);
     case ROW_PHONON: return diff_cs_phonon(m, j.Ele, x, j.Mass);
@@ -356,6 +465,33 @@ __global__ void k_row_scan(const RowJob *__restrict__ jobs, int nrows, const int
         dLr = dL_rows + drow_off[j.shell];
     }
     int64_t i = 0;
+    if (j.q32) {
+        // FP32 Simpson over the transfer-energy intervals with compensated
+        // sums; the cumulative values leave in FP64 for the normalisation
+        Kahan32 S, D;
+        float L0 = (float)Ltot0;
+        for (int64_t k = o; k < o1; ++k, ++i) {
+            const double E = Ek[k], dE = dEk[k];
+            const float La = (float)fa[k], Lb = (float)fb[k];
+            const float t2 = (float)(dE / 6.0) * ((L0 + 4.0f * La) + Lb);
+            S.add(t2);
+            // stopping power: Simpson on w * dsigma/dw over the interval
+            const float a = (float)(E + dE / 2.0), b = (float)(E + dE);
+            D.add((float)(dE / 6.0) * (((float)E * L0 + (a * 4.0f) * La) + b * Lb));
+            L0 = Lb;
+            if (hw) {
+                hw[i] = E + dE;
+                cum[i] = S.value();
+            }
+            if (dEr) {
+                dEr[i] = E;
+                dLr[i] = (((kPi * kA0) * j.Ele) * kMe) / (((j.MSHI * j.Zeff) * j.Zeff) * S.value());
+            }
+        }
+        Ltot1 = S.value();
+        ddEdx = D.value();
+        o1 = o;      // the FP64 loop below has nothing left to do
+    }
     for (int64_t k = o; k < o1; ++k, ++i) {
         double E = Ek[k], dE = dEk[k];
         double temp1 = fa[k], dL = fb[k];
@@ -750,6 +886,11 @@ int build_tables(const HostMat &h, const MatDev &md, TableSet &ts, hipStream_t s
     for (int i = 0; i < (int)gS.size(); ++i)
         for (int s = 0; s < nsh; ++s) jobs.push_back(shi_job(h, gS[i], s, 0, i));
     for (int s = 0; s < nsh; ++s) jobs.push_back(shi_job(h, h.ion.E, s, 1, 0));
+    // the precision form's quadrature for the inelastic and ion rows (the
+    // phonon rows keep the reference's)
+    if (h.o.quadrature == 1)
+        for (RowJob &j : jobs)
+            if (j.type != ROW_PHONON) j.q32 = 1;
     if (own) {
         T[TRK_TAB_E_INEL].has_rows.assign(nsh, 1);
         T[TRK_TAB_H_INEL].has_rows[h.vb] = 1;
@@ -943,6 +1084,60 @@ int build_tables(const HostMat &h, const MatDev &md, TableSet &ts, hipStream_t s
     (void)hipFree(d_fa); (void)hipFree(d_fb); (void)hipFree(d_step_row); (void)hipFree(d_outs);
     ts.launches = 12;
     ts.build_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t_start).count();
+    return 0;
+}
+
+// After tables were replaced from host arrays (trk_table_set / trk_row_set):
+// the fused paths, the logs, the row flags, the ion's inverse transfer rows
+// and the run constants derived from them, recomputed in place.
+int refresh_derived(const HostMat &h, TableSet &ts, hipStream_t st, std::string &err)
+{
+    const int nsh = h.nsh;
+    TableDev *T = ts.tab;
+    const bool own = !ts.shared_material;
+    if (own) {
+        k_fuse<<<nblk(T[TRK_TAB_E_INEL].n, 128), 128, 0, st>>>(T[TRK_TAB_E_INEL].L, T[TRK_TAB_E_INEL].n, nsh, 1, ts.el_imfp);
+        k_fuse<<<nblk(T[TRK_TAB_H_INEL].n, 128), 128, 0, st>>>(T[TRK_TAB_H_INEL].L, T[TRK_TAB_H_INEL].n, nsh, 0, ts.hole_imfp);
+        if (ts.phot_imfp)
+            k_fuse<<<nblk(T[TRK_TAB_PHOT].n, 128), 128, 0, st>>>(T[TRK_TAB_PHOT].L, T[TRK_TAB_PHOT].n, nsh, 0, ts.phot_imfp);
+    }
+    k_fuse<<<nblk(T[TRK_TAB_SHI].n, 128), 128, 0, st>>>(T[TRK_TAB_SHI].L, T[TRK_TAB_SHI].n, nsh, 0, ts.shi_path);
+    for (int k = 0; k < 6; ++k) {
+        if (!own && k != TRK_TAB_SHI) continue;
+        if (T[k].n > 0 && T[k].E && ts.ln_E[k]) k_log<<<nblk(T[k].n, 128), 128, 0, st>>>(T[k].E, T[k].n, ts.ln_E[k]);
+    }
+    struct { const double *src; int n; double *dst; bool mat; } lg[] = {
+        {ts.el_imfp, T[TRK_TAB_E_INEL].n, ts.ln_el_imfp, true}, {ts.hole_imfp, T[TRK_TAB_H_INEL].n, ts.ln_hole_imfp, true},
+        {T[TRK_TAB_E_EL].L, T[TRK_TAB_E_EL].n, ts.ln_Lel, true}, {T[TRK_TAB_H_EL].L, T[TRK_TAB_H_EL].n, ts.ln_LHel, true},
+        {ts.shi_path, T[TRK_TAB_SHI].n, ts.ln_shi_path, false}};
+    for (auto &g : lg)
+        if (g.src && g.dst && g.n > 0 && (own || !g.mat)) k_log<<<nblk(g.n, 128), 128, 0, st>>>(g.src, g.n, g.dst);
+    for (int k : {TRK_TAB_E_INEL, TRK_TAB_H_INEL, TRK_TAB_SHI}) {
+        const int64_t m = (int64_t)T[k].nsh * T[k].n;
+        if (!T[k].L || !ts.ln_L[k] || m <= 0 || (!own && k != TRK_TAB_SHI)) continue;
+        k_log<<<(unsigned)((m + 127) / 128), 128, 0, st>>>(T[k].L, m, ts.ln_L[k]);
+    }
+    for (int k : {TRK_TAB_E_INEL, TRK_TAB_H_INEL, TRK_TAB_E_EL, TRK_TAB_H_EL}) {
+        const int64_t m = T[k].row_off_h.empty() ? 0 : T[k].row_off_h.back();
+        if (!own || !T[k].hw || !ts.ln_hw[k] || m <= 0) continue;
+        k_log<<<(unsigned)((m + 127) / 128), 128, 0, st>>>(T[k].hw, m, ts.ln_hw[k]);
+        k_log<<<(unsigned)((m + 127) / 128), 128, 0, st>>>(T[k].lam, m, ts.ln_lam[k]);
+        const int64_t nslots = (int64_t)T[k].nsh * T[k].n;
+        k_row_mono<<<(unsigned)((nslots + 127) / 128), 128, 0, st>>>(T[k].row_off, nslots, T[k].lam, T[k].row_mono);
+    }
+    TableDev &td = T[TRK_TAB_SHI_DIFF];
+    HIP_OK(hipMemsetAsync(td.dNonMono, 0, sizeof(int) * nsh, st));
+    for (int sh = 0; sh < nsh; ++sh)
+        if (td.dn[sh] > 0)
+            k_inv_row<<<nblk(td.dn[sh], 256), 256, 0, st>>>(td.dL + td.doff[sh], td.dn[sh], td.dInv + td.doff[sh],
+                                                            td.dNonMono + sh);
+    HIP_OK(hipGetLastError());
+    std::vector<double> sE(T[TRK_TAB_SHI].n), sp(T[TRK_TAB_SHI].n);
+    HIP_OK(hipMemcpyAsync(sE.data(), T[TRK_TAB_SHI].E, sizeof(double) * sE.size(), hipMemcpyDeviceToHost, st));
+    HIP_OK(hipMemcpyAsync(sp.data(), ts.shi_path, sizeof(double) * sp.size(), hipMemcpyDeviceToHost, st));
+    HIP_OK(hipStreamSynchronize(st));
+    ts.shi_path0 = nfp_2d_host(sE.data(), sp.data(), (int)sE.size(), h.ion.E);
+    ts.sd_consts = false;
     return 0;
 }
 

@@ -49,7 +49,8 @@ module trekis_gpu_c
       integer(c_int32_t) :: kind_of_EMFP, CDF_elast_Zeff, kind_of_DR, plasmon_Emax, include_photons, dt_flag
       real(c_double)     :: Tim, dt, e_grid_Emax, shi_grid_Emax
       integer(c_int32_t) :: precision, batch, block, tally_replicas, gen_capacity
-      integer(c_int32_t) :: reserved(3)
+      integer(c_int32_t) :: quadrature
+      integer(c_int32_t) :: reserved(2)
    end type trk_options
 
    type, bind(C) :: trk_tallies

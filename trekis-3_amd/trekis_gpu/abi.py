@@ -96,7 +96,8 @@ class Options(ctypes.Structure):
         ("block", c_int32),
         ("tally_replicas", c_int32),
         ("gen_capacity", c_int32),
-        ("reserved", c_int32 * 3),
+        ("quadrature", c_int32),
+        ("reserved", c_int32 * 2),
     ]
 
 

@@ -425,14 +425,86 @@ def config_material(name):
 #   opt_ritchie_warm  Ritchie's extended dispersion (CS:382-385),
 #                     Nikolaev-Dmitriev charge, a 300 K target (Bose factor
 #                     of the inelastic cross sections, CS:2274-2278)
-#   opt_sg_warm       Schiwietz-Grande charge, 300 K, the reference's case
-#                     default dispersion (kind 4: free electron)
+#   opt_sg_warm       Schiwietz-Grande charge, 300 K
 #   opt_fixed_zeff    a fixed ion charge of 3.5, a 300 K metal (Bose factor of
 #                     the phonon CDF, CS:3290-3295), logarithmic time grid of
 #                     ratio 10 (the reference's own INPUT_PARAMETERS.txt:6 setting)
 OPTION_WORKLOADS = {
     "opt_plasmon_pole": ("test", 2, 1, 8.0, 0.0, 1, 1, 2.0),
     "opt_ritchie_warm": ("test", 3, 2, 8.0, 300.0, 0, 0, 1.0),
-    "opt_sg_warm": ("test", 4, 3, 8.0, 300.0, 0, 0, 1.0),
+    "opt_sg_warm": ("test", 1, 3, 8.0, 300.0, 0, 0, 1.0),
     "opt_fixed_zeff": ("test_metal", 1, 4, 3.5, 300.0, 0, 1, 10.0),
 }
+
+
+# -------------------------------------------------------------- the batch of impacts
+CONFIG4_FIXTURE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "examples", "config4_impacts.json")
+
+
+def generate_config4(n_impacts=256, n_materials=4, seed=0x5EED0004):
+    """The batching workload: n_impacts ion impacts (Z uniform in 18-92,
+    energies log-uniform in 1-20 MeV/u) on n_materials synthetic targets
+    (one element of Z 6-16 each, density uniform in 1-10 g/cm^3, band gap
+    uniform in 0-9 eV, 1-6 Drude-Lorentz valence oscillators with E0 in
+    8-40 eV and widths in 2-15 eV, a K shell at the Moseley energy
+    13.6 (Z-1)^2 eV with one oscillator and an Auger time of 2-10 fs).
+    Drawn with numpy's Philox generator from `seed`; the result is what
+    trekis_gpu/examples/config4_impacts.json holds."""
+    from .inputs import ELEMENT_MASS
+    g = np.random.Generator(np.random.Philox(seed))
+    mats = []
+    for m in range(n_materials):
+        Zt = int(g.integers(6, 17))
+        nosc = int(g.integers(1, 7))
+        E0 = np.sort(g.uniform(8.0, 40.0, nosc))
+        mats.append(dict(Z=Zt, mass=float(ELEMENT_MASS[Zt]), dens=float(g.uniform(1.0, 10.0)),
+                         gap=float(g.uniform(0.0, 9.0)),
+                         vb_osc=[(float(e), float(w), float(gm)) for e, w, gm in
+                                 zip(E0, g.uniform(0.2, 1.0, nosc), g.uniform(2.0, 15.0, nosc))],
+                         ip_k=float(round(13.6 * (Zt - 1) ** 2, 1)), auger_k=float(g.uniform(2.0, 10.0))))
+    impacts = []
+    for i in range(n_impacts):
+        Z = int(g.integers(18, 93))
+        Eu = float(np.exp(g.uniform(np.log(1.0), np.log(20.0))))
+        impacts.append(dict(Z=Z, mass=float(ELEMENT_MASS[Z]), MeV_per_u=Eu, material=int(g.integers(0, n_materials))))
+    return dict(seed=seed, materials=mats, impacts=impacts)
+
+
+def config4_material(md, layer=100.0):
+    """MaterialDesc of one synthetic target of the batching workload."""
+    Zt, gap = md["Z"], max(md["gap"], 0.1)
+    vb = Shell(gap, float(Zt - 2), [tuple(o) for o in md["vb_osc"]])
+    k = Shell(md["ip_k"], 2.0, [(1.15 * md["ip_k"], 1.0, 0.6 * md["ip_k"])], auger=md["auger_k"], shl_num=1)
+    atom = Atom(Zt, md["mass"], 1.0, [k, vb])
+    E, D = dos_free_electron(10.0) if gap < 0.2 else dos_insulator()
+    return MaterialDesc(f"synthetic Z={Zt}", [atom], dens=md["dens"], layer=layer, vsound=5000.0, E_F=10.0,
+                        el_eff_mass=1.0, hole_mass=1.0, cut_off=10.0, dos_raw_E=E, dos_raw=D)
+
+
+def config4_batch(n_impacts=None, precision=1):
+    """(materials, ions, options, run) of the batching workload read from
+    its fixture: MaterialDesc and Options per material (Mott elastic
+    scattering, photons off, 0-1000 fs in 1 fs steps), one Ion per impact
+    with the index of its material, and the radial grid (100 log bins,
+    1-1000 A over a 100 A segment)."""
+    import json
+    with open(CONFIG4_FIXTURE) as fh:
+        fx = json.load(fh)
+    imps = fx["impacts"][: n_impacts] if n_impacts else fx["impacts"]
+    mats, opts = [], []
+    for md in fx["materials"]:
+        mats.append(config4_material(md))
+        opt = abi.Options()
+        opt.kind_of_EMFP, opt.CDF_elast_Zeff, opt.kind_of_DR = 0, 0, 1
+        opt.Tim, opt.dt, opt.dt_flag = 1000.0, 1.0, 0
+        opt.include_photons = 0
+        opt.precision = precision
+        opts.append(opt)
+    ions = []
+    for im in imps:
+        ion = abi.Ion()
+        ion.Zat, ion.Mass, ion.E = im["Z"], im["mass"], im["MeV_per_u"] * 1.0e6 * im["mass"]
+        ion.Kind_Zeff, ion.fixed_Zeff, ion.Kind_ion = 0, float(im["Z"]), 0
+        ions.append((ion, im["material"]))
+    run = dict(NR=100, Rmin=1.0, Rmax=1000.0, seed=0x5EED0004)
+    return mats, opts, ions, run
