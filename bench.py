@@ -137,6 +137,9 @@ def run_leg(tg, abi, args, precision, first_iter):
     mat.prepare(lib)
     opt.gen_capacity = args.gen_capacity
     opt.precision = precision
+    # the FP32 table quadrature serves the mixed path only; FP64 keeps the
+    # reference's tables
+    opt.quadrature = args.quadrature if precision else 0
     ctx = tg.Context(mat, ion, opt)
     t0 = time.perf_counter()
     ctx.build_tables()
@@ -252,7 +255,7 @@ def main_batch(args):
                     agg_f[k] = sum(t[i][k] for t in ref["tals"] for i in idx)
                 w, pm, de, nodes = tally_errors(agg_f, agg_m, leg["V"])
                 worst = max(worst, w)
-                per[leg["mats"][mi].name] = {"max_radial_tally_rel_error": w, "tot_E_rel_diff": de,
+                per[f"material {mi} ({leg['mats'][mi].name})"] = {"max_radial_tally_rel_error": w, "tot_E_rel_diff": de,
                                              "tallies": {k: v for k, v in pm.items() if k in RADIAL}}
             compare["max_radial_tally_rel_error"] = worst
             compare["per_material"] = per
@@ -295,6 +298,9 @@ def main():
     ap.add_argument("--batch-tallies", type=int, default=1, help="config4: fetch every impact's tallies each step")
     ap.add_argument("--gen-capacity", type=int, default=1 << 27,
                     help="particles of one kind per round list (electrons, holes; photons get a quarter)")
+    ap.add_argument("--quadrature", type=int, choices=[0, 1], default=0,
+                    help="tables of the mixed leg: 0 the reference's FP64 quadrature, 1 the FP32 compensated "
+                         "quadrature (2000 geometric transfer intervals x 1000 Simpson intervals in ln q)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run a plain device-to-device copy of MIB MiB five times (the reference kernel "
                          "that scales HBM byte counters) and exit")
@@ -412,7 +418,10 @@ def main():
         "dtype": "f32 sampling / f64 state and tallies" if mixed else "f64",
         "data": "synthetic",
         "config": {"workload": f"{mat.name} target, Au 11 MeV/u ion, 0-1000 fs x 100 radial bins, "
-                               f"{B} iterations per step", "precision": args.precision, "iters_per_step": B,
+                               f"{B} iterations per step"
+                               + (", mixed leg on FP32-quadrature tables" if mixed and args.quadrature else ""),
+                   "precision": args.precision, "iters_per_step": B,
+                   "table_quadrature": "fp32 compensated" if mixed and args.quadrature else "reference fp64",
                    "events": total_ev, "events_by_kind": dict(zip(abi.EVENT_NAMES, ev.tolist())),
                    "table_build_s": main_leg["t_tables"], "ion_init_kernel_ms": main_leg["ms_shi"],
                    "ion_init_launches": main_leg["n_shi"],

@@ -106,6 +106,23 @@ class Context:
         self._check(self.lib.trk_row_get(self.h, kind, shell, i_E, dptr(hw), dptr(lam), n.value), "trk_row_get")
         return hw, lam
 
+    def set_table(self, kind, shell, L=None, dEdx=None, E=None):
+        """Overwrite one shell's mean free paths / stopping powers of a built
+        table (E, when given, must equal the built grid); finish_tables()
+        must follow before the next run."""
+        arrs = [None if a is None else np.ascontiguousarray(a, dtype=np.float64) for a in (E, L, dEdx)]
+        n = next(len(a) for a in arrs if a is not None)
+        self._check(self.lib.trk_table_set(self.h, kind, shell, *[dptr(a) if a is not None else None for a in arrs], n),
+                    "trk_table_set")
+
+    def set_row(self, kind, shell, i_E, hw, lam):
+        hw = np.ascontiguousarray(hw, dtype=np.float64)
+        lam = np.ascontiguousarray(lam, dtype=np.float64)
+        self._check(self.lib.trk_row_set(self.h, kind, shell, i_E, dptr(hw), dptr(lam), len(hw)), "trk_row_set")
+
+    def finish_tables(self):
+        self._check(self.lib.trk_tables_finish(self.h), "trk_tables_finish")
+
     def new_tallies(self, NR):
         return TallySet(self.Nt, NR, self.desc)
 

@@ -135,6 +135,9 @@ def declare_library(lib):
         "trk_table_get": (c_int, [c_void_p, c_int, c_int, DP, DP, DP, c_int32]),
         "trk_row_size": (c_int, [c_void_p, c_int, c_int, c_int32, P(c_int32)]),
         "trk_row_get": (c_int, [c_void_p, c_int, c_int, c_int32, DP, DP, c_int32]),
+        "trk_table_set": (c_int, [c_void_p, c_int, c_int, DP, DP, DP, c_int32]),
+        "trk_row_set": (c_int, [c_void_p, c_int, c_int, c_int32, DP, DP, c_int32]),
+        "trk_tables_finish": (c_int, [c_void_p]),
         "trk_time_nodes": (c_int, [c_void_p, P(c_int32)]),
         "trk_run_mc": (c_int, [c_void_p, c_int64, c_uint64, c_int64, DP, DP, c_int32, P(Tallies)]),
         "trk_run_mc_noalloc": (c_int, [c_void_p, c_int64, c_uint64, c_int64, DP, DP, c_int32, P(Tallies)]),
