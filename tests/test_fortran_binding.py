@@ -53,3 +53,26 @@ def test_fortran_module_layout_and_calls(tmp_path, lib):
     A, E0, G = np.array([1.0]), np.array([15.0]), np.array([0.6])
     lib.trk_sumrules(dp(A), dp(E0), dp(G), 1, 0.0, 1e30, ctypes.byref(a), ctypes.byref(b))
     assert int(rc) == 0 and float(ks) == a.value and float(fs) == b.value
+
+
+REF_OBJECTS = "/root/reference/Source_files/Objects.f90"
+
+
+@pytest.mark.skipif(not (os.path.exists(FLANG) and os.path.exists(REF_OBJECTS)),
+                    reason="needs the Fortran compiler and the reference's Objects.f90")
+def test_bridge_compiles_against_the_reference_types(lib):
+    """trekis_gpu_bridge.f90 (do_Monte_Carlo with the reference's argument
+    list, Monte_Carlo.f90:39-92) compiles with the reference's own
+    Objects.f90 types, and a driver that calls it as MAIN does links."""
+    d = os.path.join(ROOT, "trekis-3_amd", "fortran")
+    subprocess.run(["make", "-C", d], check=True, capture_output=True)
+    assert os.path.exists(os.path.join(d, "_build", "bridge_driver"))
+    mod = open(os.path.join(d, "trekis_gpu_bridge.f90")).read()
+    ref = open("/root/reference/Source_files/Monte_Carlo.f90").read()
+    import re
+
+    def args(text):
+        i = text.index("subroutine do_Monte_Carlo(")
+        j = text.index(")", i)
+        return [a.strip().lower() for a in re.sub(r"&\s*\n\s*", "", text[i + 26:j]).split(",")]
+    assert args(mod) == args(ref)

@@ -2,11 +2,13 @@
 trk_row_set, trk_tables_finish behind trekis_gpu.cache.seed_from_cache).
 
 A context builds its tables and writes them in the reference's layout; a
-second context of the same material builds different tables (the FP32
-quadrature), is seeded from those files, and then holds the first one's
-tables bit for bit -- every mean free path, stopping power of the ion and
-cumulative row -- and runs the same histories: every tally equal bit for bit
-to the first context's run.  A run between a set and the finish is refused.
+second context of the same material builds its tables, has every mean free
+path and every cumulative row overwritten with other values (scaled by 1.5)
+so that its runs differ, is seeded from the files, and then holds the first
+one's tables bit for bit -- every mean free path, stopping power of the ion
+and cumulative row -- and runs the same histories: every tally equal bit for
+bit to the first context's run.  A run between a set and the finish is
+refused.
 """
 import numpy as np
 import pytest
@@ -43,13 +45,25 @@ def _all_tables(ctx, nsh):
 @pytest.mark.parametrize("name", ["test_cdf", "test"])
 def test_seeded_tables_and_run_equal_the_writer(name, tmp_path):
     a, mat, run = _ctx(name, 0)
-    b, _, _ = _ctx(name, 1)
+    b, _, _ = _ctx(name, 0)
     nsh = sum(mat.n_shells)
     say(f"[cache] {name}: writing the reference's cache files")
     names = cache.write_cache(a, str(tmp_path))
+    R, V = tg.radial_grid(run["NR"], run["Rmin"], run["Rmax"], mat.layer)
+    nmc = 4
+    ra = a.run_mc(nmc, run["seed"], R, V)
+    # other tables in b: every mean free path and cumulative row scaled
+    for (kind, s), (E, L, D) in _all_tables(b, nsh).items():
+        b.set_table(kind, s, L=1.5 * L)
+    for kind, s, i, _ in cache._row_files(b, names, str(tmp_path)):
+        hw, lam = b.row(kind, s, i)
+        b.set_row(kind, s, i, hw, 1.5 * lam)
+    b.finish_tables()
     ta, tb = _all_tables(a, nsh), _all_tables(b, nsh)
-    assert any(not np.array_equal(ta[k][1], tb[k][1]) for k in ta), "the FP32 tables should differ"
-    say(f"[cache] {name}: seeding a context built with the FP32 quadrature")
+    assert all(not np.array_equal(ta[k][1], tb[k][1]) for k in ta if np.isfinite(ta[k][1]).any())
+    rx = b.run_mc(nmc, run["seed"], R, V)
+    assert not all(np.array_equal(ra[f], rx[f]) for f in ra.arrays), "scaled tables should change the run"
+    say(f"[cache] {name}: seeding a context whose tables were scaled")
     cache.seed_from_cache(b, str(tmp_path))
     tb = _all_tables(b, nsh)
     for k in ta:
@@ -64,9 +78,6 @@ def test_seeded_tables_and_run_equal_the_writer(name, tmp_path):
         assert np.array_equal(ha, hb) and np.array_equal(la, lb), (kind, s, i)
         rows += 1
     say(f"[cache] {name}: {len(ta)} tables and {rows} rows equal")
-    R, V = tg.radial_grid(run["NR"], run["Rmin"], run["Rmax"], mat.layer)
-    nmc = 4
-    ra = a.run_mc(nmc, run["seed"], R, V)
     rb = b.run_mc(nmc, run["seed"], R, V)
     for f in ra.arrays:
         assert np.array_equal(ra[f], rb[f]), f
