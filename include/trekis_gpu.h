@@ -197,7 +197,7 @@ int  trk_row_get(const trk_ctx *ctx, int kind, int shell, int32_t i_E, double *h
 /* Tables read from the reference's cache files instead of computed
  * (Analytical_IMFPs.f90:259-383 reads OUTPUT_*_IMFPs*.dat through
  * read_write_IMFP 585-630, and 663-711 / 745-790 / 1450-1510 / 1540-1600 the
- * diff_CS/*.dat rows through read(FN_diff,*)).  After trk_build_tables, each
+ * row files of the diff_CS folder through read(FN_diff,*)).  After trk_build_tables, each
  * call overwrites one shell's mean free path (L) and stopping power (dEdx)
  * of a table kind -- E, when not NULL, must equal the built grid bit for bit
  * -- or one cumulative row (hw, lambda_cum) of the same length as built;

@@ -76,6 +76,9 @@ double or_mott_theta(const or_ctx *c, int atom, double Ee, double RN, double *nu
 double or_mott_W(const or_ctx *c, int atom, double Ee, double theta, double M_eff);
 double or_cos_theta_W(const or_ctx *c, double E, double W, double M_in);
 int    or_auger_pick(const or_ctx *c, double NRG, double frac, int second);
+/* angle composition (New_Angles_both) and the hole's gap walk (tests) */
+void   or_new_angles(double phi0, double theta0, double theta, double psi, double *phi1, double *theta1);
+double or_gap_walk(const or_ctx *c, double Ehole, int down);
 double or_constant(int which);
 
 /* Monte Carlo (do_Monte_Carlo in the reference's global time order).  If

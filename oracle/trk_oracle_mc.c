@@ -392,6 +392,11 @@ static void new_angles(double phi0, double theta0, double theta, double psi, dou
     *theta1 = t;
 }
 
+void or_new_angles(double phi0, double theta0, double theta, double psi, double *phi1, double *theta1)
+{
+    new_angles(phi0, theta0, theta, psi, phi1, theta1);
+}
+
 /* Electron_recieves_E, MC:1653-1716 */
 static double receives_E(or_mc *S, double dE, int shell, uint64_t id, uint32_t *nd)
 {
@@ -803,24 +808,40 @@ static void electron_event(or_mc *S, int idx)
 }
 
 /* ---------------------------------------------------------------- holes */
+/* The gap walk of check_hole_parameters, MC:696-719: a hole energy whose
+ * DOS bracket is empty (< 1e-4) moves to the nearest nonempty level below
+ * (down = 1, the excess goes to the electron) or above (down = 0, to the
+ * lattice); the walk stops at the ends of the DOS array.  Returns 1 and the
+ * level in *Eloc when the bracket was empty. */
+static int gap_walk(const or_ctx *c, double Ehole, int down, double *Eloc)
+{
+    int m = or_find_mono_1d(c->dosE, c->m.n_dos, Ehole);
+    if (!(c->dos[m - 1] < 1.0e-4)) return 0;
+    if (down) {
+        while (m > 1 && c->dos[m - 1] < 1.0e-4) m = m - 1;
+    } else {
+        while (m < c->m.n_dos && c->dos[m - 1] < 1.0e-4) m = m + 1;
+    }
+    *Eloc = c->dosE[m - 1];
+    return 1;
+}
+
+double or_gap_walk(const or_ctx *c, double Ehole, int down)
+{
+    double Eloc = Ehole;
+    gap_walk(c, Ehole, down, &Eloc);
+    return Eloc;
+}
+
 /* check_hole_parameters, MC:682-721 (walks clamped to the DOS array) */
 static double check_hole(or_mc *S, double Eel, double *dE, int with_el, int kel)
 {
-    or_ctx *c = S->c;
     double Ehole = Eel - *dE;
-    int m = or_find_mono_1d(c->dosE, c->m.n_dos, Ehole);
-    if (c->dos[m - 1] < 1.0e-4) {
-        if (with_el) {
-            while (m > 1 && c->dos[m - 1] < 1.0e-4) m = m - 1;
-            double Eloc = c->dosE[m - 1];
-            S->el[kel].E = S->el[kel].E + (Ehole - Eloc);
-            Ehole = Eloc;
-        } else {
-            while (m < c->m.n_dos && c->dos[m - 1] < 1.0e-4) m = m + 1;
-            double Eloc = c->dosE[m - 1];
-            *dE = *dE + (Ehole - Eloc);
-            Ehole = Eloc;
-        }
+    double Eloc;
+    if (gap_walk(S->c, Ehole, with_el, &Eloc)) {
+        if (with_el) S->el[kel].E = S->el[kel].E + (Ehole - Eloc);
+        else *dE = *dE + (Ehole - Eloc);
+        Ehole = Eloc;
     }
     return Ehole;
 }

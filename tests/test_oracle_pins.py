@@ -1,0 +1,240 @@
+"""Oracle functions pinned by physics the reference's formulas stand on,
+computed here from CODATA constants and first principles rather than from
+the reference's expressions:
+
+* screening_parameter (Cross_sections.f90:3598-3613): the prefactor 1.7e-5
+  Z^(2/3) (1 - beta^2)/beta^2 is the Thomas-Fermi screening
+  (hbar / (2 p a_TF))^2, a_TF = 0.88534 a0 Z^(-1/3), with p the electron's
+  relativistic momentum; the rest is Moliere's correction
+  1.13 + 3.76 (alpha Z / beta)^2 sqrt(tau / (1 + tau)) (Salvat et al.,
+  PENELOPE; Bielajew).  A power of Z, a non-relativistic beta or a missing
+  alpha fails it.
+* Mott elastic mean free paths (Elastic_cross_section / Atomic_elastic_sigma,
+  CS:2939-2962, 3495-3514): 1 / (n sum_i x_i sigma_i) with sigma_i the
+  screened-Rutherford cross section integrated here over the solid angle
+  numerically, Z(Z+1) (e^2 / 2E)^2 / (1 - mu + 2 eta)^2, e^2 = 14.3996 eV A.
+* the transfer limit of the phonon-CDF elastic tables (Tot_EMFP,
+  CS:2966-3025): the integration runs to the larger of the head-on two-body
+  transfer (found here by scanning the centre-of-mass angle of an elastic
+  collision with the mean target mass) and 3 Debye energies (hbar v_s k_D,
+  k_D = (6 pi^2 n)^(1/3)), but not past the electron's energy: the last
+  interval of every cumulative row straddles that limit.
+* the photon mean free path (Tot_Phot_IMFP, CS:851-853): the attenuation
+  length c / (omega Im[-1/eps(omega, 0)]) of a photon of energy E = hbar
+  omega, with Im summed over the shell's oscillators (loss function pinned
+  by its sum rules elsewhere); infinite (1e30) below the shell's edge.
+* New_Angles_both (Monte_Carlo.f90:1328-1360): the folded (theta1, phi1)
+  lie in [0, pi] x [0, 2 pi] and point along the same unit vector as the
+  unfolded composition; psi = 0 turns theta only, and at theta0 = pi/2 the
+  azimuth does not move.
+* the valence-hole gap walk (check_hole_parameters, MC:696-719): brute
+  force over a small DOS with empty levels -- the nearest nonempty level
+  below (hole-electron) or above (hole-lattice) the bracket of the energy.
+"""
+import ctypes
+import math
+
+import numpy as np
+import pytest
+from scipy import integrate
+
+import trekis_gpu as tg
+from trekis_gpu import abi
+from oracle import Oracle
+
+# CODATA (2018) values, independent of the oracle's own constants
+HBARC = 1973.269804          # eV A
+MEC2 = 510998.95             # eV
+A0 = 0.529177210903          # A
+ALPHA = 7.2973525693e-3
+E2 = 14.399645               # e^2 / (4 pi eps0), eV A
+HBAR = 1.054571817e-34       # J s
+EV = 1.602176634e-19         # J
+AMU_ME = 1836.15267343       # proton mass / electron mass
+
+
+@pytest.fixture(scope="module")
+def oxide(lib):
+    mat, ion, opt, run = tg.config_material("test")
+    mat.prepare(lib)
+    o = Oracle(mat, ion, opt)
+    o.build_tables(threads=8)
+    return o, mat
+
+
+@pytest.fixture(scope="module")
+def cdf(lib):
+    mat, ion, opt, run = tg.config_material("test_cdf")
+    mat.prepare(lib)
+    o = Oracle(mat, ion, opt)
+    o.build_tables(threads=8)
+    return o, mat
+
+
+def _moliere(Z, E):
+    tau = E / MEC2
+    b2 = 1.0 - 1.0 / (1.0 + tau) ** 2
+    pc = math.sqrt(E * (E + 2.0 * MEC2))
+    a_tf = 0.88534 * A0 * Z ** (-1.0 / 3.0)
+    eta_tf = (HBARC / (2.0 * pc * a_tf)) ** 2
+    return eta_tf, 1.13 + 3.76 * (ALPHA * Z) ** 2 / b2 * math.sqrt(tau / (1.0 + tau))
+
+
+@pytest.mark.parametrize("Ee", [20.0, 300.0, 5000.0, 1.0e5, 1.0e6])
+def test_screening_is_thomas_fermi_with_moliere_correction(oxide, Ee):
+    o, mat = oxide
+    for atom in range(mat.n_atoms):
+        Z = mat.atoms[atom].Z
+        nu = ctypes.c_double()
+        o.lib.or_mott_theta(o.h, atom, Ee, 0.5, ctypes.byref(nu))
+        eta_tf, corr = _moliere(Z, Ee)
+        # 1.7e-5 rounds the Thomas-Fermi constant 1.6984e-5 (+0.09 %)
+        assert nu.value / (eta_tf * corr) == pytest.approx(1.0009, abs=1.5e-3), (Z, Ee)
+
+
+def _sigma_rutherford(Z, E, eta):
+    """Screened Rutherford cross section [A^2] integrated numerically over mu."""
+    pref = Z * (Z + 1.0) * (E2 / (2.0 * E)) ** 2
+    f = lambda mu: pref / (1.0 - mu + 2.0 * eta) ** 2
+    pts = [1.0 - 10.0 * eta] if eta < 0.1 else None
+    val = integrate.quad(f, -1.0, 1.0, points=pts, limit=400, epsabs=0.0, epsrel=1e-11)[0]
+    return 2.0 * math.pi * val
+
+
+def test_mott_mean_free_path_is_the_integrated_screened_rutherford(oxide):
+    o, mat = oxide
+    E, L, _ = o.table(abi.TAB_E_EL, 0)
+    x = np.array([a.pers for a in mat.atoms])
+    n_A3 = mat.At_Dens * 1e-24
+    checked = 0
+    for i in range(0, len(E), 23):
+        Ee = E[i]
+        tot = 0.0
+        for a, xa in zip(mat.atoms, x):
+            # the screening of this table: Thomas-Fermi with the
+            # non-relativistic beta^2 = 2E / mc^2 (Atomic_elastic_sigma)
+            b2 = 2.0 * Ee / MEC2
+            eta = 1.7e-5 * a.Z ** (2.0 / 3.0) * (1.0 - b2) / b2 * (
+                1.13 + 3.76 * (a.Z / 137.0) ** 2 / b2 * math.sqrt(Ee / (Ee + MEC2)))
+            tot += xa * _sigma_rutherford(a.Z, Ee, eta)
+        expect = 1.0 / (n_A3 * tot / x.sum())
+        assert L[i] == pytest.approx(expect, rel=2e-6), (Ee, L[i], expect)
+        checked += 1
+    assert checked > 10
+
+
+def _head_on_transfer(E, m, M):
+    """Largest energy an elastic two-body collision of a mass m of kinetic
+    energy E hands a mass M at rest: scan of the centre-of-mass angle."""
+    v = math.sqrt(2.0 * E / m)
+    vcm = m * v / (m + M)
+    best = 0.0
+    for th in np.linspace(0.0, math.pi, 2001):
+        # the target leaves with the centre-of-mass velocity plus its own
+        # CM velocity (magnitude vcm) turned by th
+        vx = vcm - vcm * math.cos(th)
+        vy = vcm * math.sin(th)
+        best = max(best, 0.5 * M * (vx * vx + vy * vy))
+    return best
+
+
+def test_phonon_rows_end_at_the_transfer_limit(cdf):
+    o, mat = cdf
+    n_m3 = mat.At_Dens * 1e6
+    k_d = (6.0 * math.pi ** 2 * n_m3) ** (1.0 / 3.0)
+    E_debye = HBAR * mat.vsound * k_d / EV
+    M = AMU_ME * sum(a.mass * a.pers for a in mat.atoms) / sum(a.pers for a in mat.atoms)   # in electron masses
+    seen = {"debye": 0, "collision": 0, "energy": 0}
+    cap = 1 << 16
+    hw, lam = np.zeros(cap), np.zeros(cap)
+    L, D, n = ctypes.c_double(), ctypes.c_double(), ctypes.c_int32()
+    for Ee in np.geomspace(0.05, 1.0e5, 61):
+        coll = _head_on_transfer(Ee, 1.0, M)
+        lim = max(coll, 3.0 * E_debye)
+        which = "collision" if coll > 3.0 * E_debye else "debye"
+        if Ee < lim:
+            lim, which = Ee, "energy"
+        rc = o.lib.or_inel_point(o.h, abi.TAB_E_EL, 0, float(Ee), ctypes.byref(L), ctypes.byref(D),
+                                 hw.ctypes.data_as(abi.DP), lam.ctypes.data_as(abi.DP), cap, ctypes.byref(n))
+        assert rc == 0 and 2 <= n.value <= cap
+        k = n.value - 1
+        assert hw[k - 1] <= lim * (1 + 1e-9) < hw[k] * (1 + 2e-9), (Ee, which, hw[k - 1], lim, hw[k])
+        seen[which] += 1
+    assert min(seen.values()) > 0, seen
+
+
+def test_photon_path_is_the_attenuation_length(cdf):
+    o, mat = cdf
+    m = o.mat
+    osc_off = np.concatenate([[0], np.cumsum([m.n_osc[s] for s in range(sum(mat.n_shells))])])
+    E, _, _ = o.table(abi.TAB_PHOT, 0)
+    checked = 0
+    for s in range(sum(mat.n_shells)):
+        _, L, _ = o.table(abi.TAB_PHOT, s)
+        for i in range(0, len(E), 17):
+            if E[i] < m.Ip[s]:
+                assert L[i] == 1e30
+                continue
+            im = sum(o.lib.or_loss_func(o.h, 1, m.osc_A[k], m.osc_E0[k], m.osc_Gamma[k], E[i], 0.0)
+                     for k in range(osc_off[s], osc_off[s + 1]))
+            omega_over_c = E[i] / HBARC                       # 1/A
+            assert L[i] * omega_over_c * im == pytest.approx(1.0, rel=1e-6), (s, E[i])
+            checked += 1
+    assert checked > 20
+
+
+def _unit(theta, phi):
+    return np.array([math.sin(theta) * math.cos(phi), math.sin(theta) * math.sin(phi), math.cos(theta)])
+
+
+def test_angle_composition(lib):
+    import oracle as orc
+    olib = orc.load()
+    p1, t1 = ctypes.c_double(), ctypes.c_double()
+    g = np.random.default_rng(11)
+    for _ in range(4000):
+        phi0, theta0 = g.uniform(0, 2 * math.pi), g.uniform(0, math.pi)
+        theta, psi = g.uniform(0, math.pi), g.uniform(0, 2 * math.pi)
+        olib.or_new_angles(phi0, theta0, theta, psi, ctypes.byref(p1), ctypes.byref(t1))
+        assert 0.0 <= t1.value <= math.pi and 0.0 <= p1.value <= 2 * math.pi
+        raw = _unit(theta0 + theta * math.cos(psi), phi0 + theta * math.cos(theta0) * math.sin(psi))
+        assert np.allclose(_unit(t1.value, p1.value), raw, atol=1e-12)
+    # psi = 0: the polar angle alone turns (folded back into [0, pi])
+    olib.or_new_angles(1.0, 0.3, 0.4, 0.0, ctypes.byref(p1), ctypes.byref(t1))
+    assert t1.value == pytest.approx(0.7) and p1.value == pytest.approx(1.0)
+    olib.or_new_angles(1.0, 2.9, 0.5, 0.0, ctypes.byref(p1), ctypes.byref(t1))
+    assert t1.value == pytest.approx(2 * math.pi - 3.4) and p1.value == pytest.approx(1.0 - math.pi + 2 * math.pi)
+    # theta0 = pi/2: cos(theta0) = 0, the azimuth stays
+    olib.or_new_angles(2.0, math.pi / 2, 0.3, 1.2, ctypes.byref(p1), ctypes.byref(t1))
+    assert p1.value == pytest.approx(2.0, abs=1e-15)
+
+
+def test_gap_walk_brute_force(lib):
+    """Small DOS with empty levels: the walk lands on the nearest nonempty
+    level below (down) or above (up) the bracket of the energy."""
+    mat, ion, opt, run = tg.config_material("test")
+    g = np.random.default_rng(5)
+    for trial in range(40):
+        n = int(g.integers(6, 14))
+        E = np.cumsum(g.uniform(0.5, 2.0, n))
+        D = g.uniform(0.1, 1.0, n)
+        D[g.random(n) < 0.45] = 0.0
+        D[int(g.integers(0, n))] = 0.5           # at least one level nonempty
+        mat.dos_raw_E, mat.dos_raw = E, D
+        mat.prepare(lib)
+        o = Oracle(mat, ion, opt)
+        dE = np.array(o.mat.dos_E[:o.mat.n_dos])
+        dd = np.array(o.mat.dos_DOS[:o.mat.n_dos])
+        N = len(dE)
+        for Eh in g.uniform(dE[0] - 1.0, dE[-1] + 1.0, 30):
+            # bracket: find_in_array_monoton (pinned elsewhere), 1-based
+            m = int(o.lib.or_find_mono_1d(dE.ctypes.data_as(abi.DP), N, float(Eh)))
+            empty = dd[m - 1] < 1e-4
+            nonempty = np.flatnonzero(dd >= 1e-4) + 1
+            below = [k for k in nonempty if k <= m]
+            above = [k for k in nonempty if k >= m]
+            want_down = dE[(max(below) if below else 1) - 1] if empty else Eh
+            want_up = dE[(min(above) if above else N) - 1] if empty else Eh
+            assert o.lib.or_gap_walk(o.h, float(Eh), 1) == want_down, (trial, Eh)
+            assert o.lib.or_gap_walk(o.h, float(Eh), 0) == want_up, (trial, Eh)
+        o.close()
