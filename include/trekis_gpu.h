@@ -273,6 +273,10 @@ int  trk_batch_error(const trk_batch *b, char *buf, size_t n);
  * (Out_Ee_vs_E, Out_Eh_vs_E, Out_theta, Out_theta_h, Out_diff_coeff) must
  * be NULL, and n_events is not filled (see trk_batch_timing). */
 int  trk_batch_run(trk_batch *b, uint64_t seed, int64_t first_iter, trk_tallies *const *outs);
+/* The tallies of the last trk_batch_run (or tuning run), added into outs as
+ * trk_batch_run's outs would be: a caller that runs with outs = NULL keeps
+ * the device-side work and fetches (over PCIe) only the runs it wants. */
+int  trk_batch_fetch(trk_batch *b, trk_tallies *const *outs);
 /* Event trace of the next runs, records as in trk_set_trace. */
 int  trk_batch_set_trace(trk_batch *b, double *buf, int64_t cap_records, int64_t *n_out);
 /* Autotuner: times the batch (seed, first_iter) with every candidate of

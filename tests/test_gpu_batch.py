@@ -108,11 +108,15 @@ def test_batch_fp64_each_impact_equals_its_oracle_run():
             scale = max(float(np.abs(r).max()), 1e-300)
             assert np.allclose(a, r, rtol=rt, atol=rt * scale), (i, k)
     assert (total_events == ev).all()
-    # a second run replays the captured graph: same results
+    # a second run replays the captured graph: same results; a run without
+    # host tallies leaves them on the device for trk_batch_fetch
     tal2 = b.run(seed, first)
+    b.run(seed, first, with_tallies=False)
+    tal3 = b.fetch()
     for i in range(len(imps)):
         for k in BATCH_TALLIES:
             assert np.array_equal(tal[i][k], tal2[i][k]), (i, k)
+            assert np.array_equal(tal[i][k], tal3[i][k]), (i, k)
 
 
 def test_batch_mixed_within_1pct_of_fp64():

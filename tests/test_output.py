@@ -62,3 +62,68 @@ def test_output_tables_follow_save_output():
     TL = tab["Radial_Lattice_temperature[K].txt"]
     assert np.allclose(TL / lat, 11604.0 * 2.0 / 3.0 / (mat.At_Dens * 1e-24), rtol=1e-13)
     assert tab[("holes_density", 1, 0)][3, 4] == n["Out_nh"][4, 3, 1, 0] * 1.0e24
+
+
+def _read_matrix(path):
+    """(header tokens, row labels, values) of a radius x time file."""
+    from trekis_gpu.cache import parse_real
+    lines = open(path).read().splitlines()
+    rows = [[parse_real(x) for x in l.split()] for l in lines[1:] if l.strip()]
+    return lines[0], np.array([r[0] for r in rows]), np.array([r[1:] for r in rows])
+
+
+def test_output_files_follow_save_output(tmp_path):
+    """write_output() writes Save_output's files (SO:536-1138): names with
+    element and shell designators ('Radial_Al_L-shell_holes_...',
+    'Radial_Valence_holes_...'), time headers in f10.2 with '[fs]', rows
+    labelled in f9.1 and values in E25.16 that read back as the normalised
+    numbers to 16 digits; the VB temperature only for designator 63, the
+    hole-energy files split into potential and kinetic for the VB."""
+    from trekis_gpu import output
+    t = _fake_tallies(Nt=5, NR=4)
+    E, D = configs.dos_insulator()
+    mat = inputs.read_cdf(TWO_ELEMENTS, layer=100.0, dos=(E, D))
+    times = [10.0, 20.0, 30.0, 40.0, 50.0]
+    R = [1.0, 2.5, 10.0, 100.0]
+    dos_E = np.linspace(-10.0, 0.0, 7)
+    names = output.write_output(str(tmp_path), t, 4, mat, times, R, include_photons=True, dos_E=dos_E,
+                                hole_mass=1.0)
+    n = output.normalise(t, 4)
+    want = {"Electrons_theta_distribution.txt", "VB_holes_theta_distribution.txt", "Total_numbers.txt",
+            "Hole_mean_diffusion_coefficient.txt", "Total_energies.txt", "Radial_electron_density[1_cm^-3].txt",
+            "Radial_photon_energy[eV_A^-3].txt", "Electron_distribution_vs_E[1_eV].txt",
+            "VB_holes_distribution_vs_E[1_eV].txt", "Radial_Lattice_temperature[K].txt",
+            "Radial_Al_L-shell_holes_density[1_cm^-3].txt", "Radial_Al_L-shell_holes_energy[eV_A^-3].txt",
+            "Radial_Valence_holes_density[1_cm^-3].txt", "Radial_Valence_holes_pot_energy[eV_A^-3].txt",
+            "Radial_Valence_holes_kin_energy[eV_A^-3].txt", "Radial_Valence_holes_temperature[K].txt",
+            "Radial_O_K-shell_holes_density[1_cm^-3].txt"}
+    assert want <= set(names) and all((tmp_path / x).exists() for x in names)
+    head, lab, val = _read_matrix(tmp_path / "Radial_electron_density[1_cm^-3].txt")
+    assert head == "#Radius[A] " + "".join(f"{x:10.2f}[fs]   " for x in times) + " "
+    assert lab.tolist() == R
+    assert np.allclose(val, (n["Out_ne"] * 1e24).T, rtol=2e-16 * 8, atol=0)
+    first = open(tmp_path / "Radial_electron_density[1_cm^-3].txt").read().splitlines()[1]
+    assert first[:9] == "      1.0" and len(first) == 9 + 25 * len(times) + 1
+    _, lab, val = _read_matrix(tmp_path / "Radial_Valence_holes_kin_energy[eV_A^-3].txt")
+    assert np.allclose(val, n["Out_Ehkin"][:, :, 0, 1].T, rtol=1e-15, atol=0)
+    _, lab, val = _read_matrix(tmp_path / "Radial_Al_L-shell_holes_energy[eV_A^-3].txt")
+    assert np.allclose(val, n["Out_Eh"][:, :, 0, 0].T, rtol=1e-15, atol=0)
+    # spectrum rows are labelled by Out_R (SO:867), its header has no unit
+    lines = open(tmp_path / "Electron_distribution_vs_E[1_eV].txt").read().splitlines()
+    assert lines[0] == "#Energy[eV]   Spectrum[arb.units]"
+    assert lines[1] == "#time[fs]:  " + "".join(f"{x:10.2f}" for x in times) + " "
+    # theta: 180 rows labelled 1..180
+    _, lab, val = _read_matrix(tmp_path / "VB_holes_theta_distribution.txt")
+    assert lab.tolist() == [float(i) for i in range(1, 181)] and np.allclose(val, n["Out_theta_h"][:5].T, rtol=1e-15)
+    assert open(tmp_path / "VB_holes_theta_distribution.txt").readline().startswith("Angle[deg] ")
+    # totals: six columns with photons, emitted quantities zero
+    rows = np.loadtxt(tmp_path / "Total_numbers.txt", comments="#")
+    assert rows.shape == (5, 6) and (rows[:, 2] == 0).all() and np.allclose(rows[:, 5], n["Out_tot_Nphot"])
+    head = open(tmp_path / "Total_energies.txt").readline()
+    assert "Al_L-shell[eV]" in head and "Valence[eV]" in head and "O_K-shell[eV]" in head
+    rows = np.loadtxt(tmp_path / "Total_energies.txt", comments="#")
+    assert rows.shape == (5, 5 + 3) and np.allclose(rows[:, 5], n["Out_E_h"][:, 0, 0])
+    # no hole transport: the diffusion file exists and is empty
+    output.write_output(str(tmp_path / "heavy"), t, 4, mat, times, R, include_photons=False, hole_mass=1e30)
+    assert (tmp_path / "heavy" / "Hole_mean_diffusion_coefficient.txt").read_text() == ""
+    assert not (tmp_path / "heavy" / "Radial_photon_density[1_cm^-3].txt").exists()

@@ -85,6 +85,7 @@ struct trk_batch {
     hipGraphExec_t exec = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     double last_ms = 0.0;
+    bool has_run = false;          // a run's finished tally planes are on the device
     int64_t last_events[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     int rounds = 0;
     std::vector<trk::TuneEntry> tune_log;
@@ -377,6 +378,8 @@ static int batch_capture(trk_batch *b)
     return 0;
 }
 
+static int batch_fetch(trk_batch *b, trk_tallies *const *outs);
+
 // One replay of the batch graph; host-side tallies only when outs is given.
 static int batch_launch(trk_batch *b, uint64_t seed, int64_t first_iter, trk_tallies *const *outs)
 {
@@ -422,15 +425,41 @@ static int batch_launch(trk_batch *b, uint64_t seed, int64_t first_iter, trk_tal
         BHIP(hipMemcpy(b->trace_host, b->d_trace, sizeof(double) * 12 * ncopy, hipMemcpyDeviceToHost));
         if (b->trace_n_out) *b->trace_n_out = (int64_t)tn;
     }
-    if (outs) {
-        const int n = (int)b->impacts.size();
-        for (int i = 0; i < n; ++i) {
-            if (!outs[i]) continue;
-            trk_ctx *c = b->impacts[i];
-            const Tally &T = b->groups[b->group_of[i]].T;
-            const int rc = finish_tallies_host(c, T, b->plane_of[i], b->V.data(), b->NR, outs[i], st);
-            if (rc) return berr(b, rc, "tallies of impact %d: %s", i, c->err.c_str());
-        }
+    b->has_run = true;
+    return outs ? batch_fetch(b, outs) : 0;
+}
+
+// The host tallies of every impact of the last run, from its finished planes.
+static int batch_fetch(trk_batch *b, trk_tallies *const *outs)
+{
+    const int n = (int)b->impacts.size();
+    for (int i = 0; i < n; ++i) {
+        if (!outs[i]) continue;
+        trk_ctx *c = b->impacts[i];
+        const Tally &T = b->groups[b->group_of[i]].T;
+        const int rc = finish_tallies_host(c, T, b->plane_of[i], b->V.data(), b->NR, outs[i], b->stream);
+        if (rc) return berr(b, rc, "tallies of impact %d: %s", i, c->err.c_str());
+    }
+    return 0;
+}
+
+static int check_outs(trk_batch *b, trk_tallies *const *outs)
+{
+    const int n = (int)b->impacts.size();
+    for (int i = 0; i < n; ++i) {
+        trk_tallies *o = outs[i];
+        if (!o) continue;
+        const trk_ctx *c = b->impacts[i];
+        if (o->Nt != c->hm.Nt || o->NR != b->NR)
+            return berr(b, TRK_ERR_ARG, "tallies of impact %d have Nt=%d NR=%d, the impact has %d and %d", i, o->Nt,
+                        o->NR, c->hm.Nt, b->NR);
+        if (o->Out_Ee_vs_E || o->Out_Eh_vs_E || o->Out_theta || o->Out_theta_h || o->Out_diff_coeff)
+            return berr(b, TRK_ERR_ARG, "the per-iteration normalised tallies (spectra, angles, diffusion) are "
+                                        "made by trk_run_mc, not by batches: pass NULL for them");
+        int ms = 0;
+        for (int a = 0; a < c->hm.m.n_atoms; ++a) ms = c->hm.m.n_shells[a] > ms ? c->hm.m.n_shells[a] : ms;
+        if ((o->Out_nh || o->Out_Eh || o->Out_Ehkin || o->Out_E_h) && o->max_shells < ms)
+            return berr(b, TRK_ERR_ARG, "tallies of impact %d have max_shells=%d < %d", i, o->max_shells, ms);
     }
     return 0;
 }
@@ -610,23 +639,19 @@ int trk_batch_run(trk_batch *b, uint64_t seed, int64_t first_iter, trk_tallies *
             return berr(b, TRK_ERR_STATE, "impact %d: tables were set after the batch was made; finish them and make "
                                           "the batch again", i);
     }
-    if (outs)
-        for (int i = 0; i < n; ++i) {
-            trk_tallies *o = outs[i];
-            if (!o) continue;
-            const trk_ctx *c = b->impacts[i];
-            if (o->Nt != c->hm.Nt || o->NR != b->NR)
-                return berr(b, TRK_ERR_ARG, "tallies of impact %d have Nt=%d NR=%d, the impact has %d and %d", i, o->Nt,
-                            o->NR, c->hm.Nt, b->NR);
-            if (o->Out_Ee_vs_E || o->Out_Eh_vs_E || o->Out_theta || o->Out_theta_h || o->Out_diff_coeff)
-                return berr(b, TRK_ERR_ARG, "the per-iteration normalised tallies (spectra, angles, diffusion) are "
-                                            "made by trk_run_mc, not by batches: pass NULL for them");
-            int ms = 0;
-            for (int a = 0; a < c->hm.m.n_atoms; ++a) ms = c->hm.m.n_shells[a] > ms ? c->hm.m.n_shells[a] : ms;
-            if ((o->Out_nh || o->Out_Eh || o->Out_Ehkin || o->Out_E_h) && o->max_shells < ms)
-                return berr(b, TRK_ERR_ARG, "tallies of impact %d have max_shells=%d < %d", i, o->max_shells, ms);
-        }
+    if (outs) {
+        const int rc = check_outs(b, outs);
+        if (rc) return rc;
+    }
     return batch_launch(b, seed, first_iter, outs);
+}
+
+int trk_batch_fetch(trk_batch *b, trk_tallies *const *outs)
+{
+    if (!b || !outs) return TRK_ERR_ARG;
+    if (!b->has_run) return berr(b, TRK_ERR_STATE, "trk_batch_run must run before its tallies are fetched");
+    const int rc = check_outs(b, outs);
+    return rc ? rc : batch_fetch(b, outs);
 }
 
 int trk_batch_tune(trk_batch *b, uint64_t seed, int64_t first_iter)

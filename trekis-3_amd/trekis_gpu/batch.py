@@ -36,6 +36,7 @@ def _declare(lib):
         "trk_batch_error": (c_int, [c_void_p, c_char_p, c_size_t]),
         "trk_batch_set_trace": (c_int, [c_void_p, abi.DP, c_int64, P(c_int64)]),
         "trk_batch_run": (c_int, [c_void_p, c_uint64, c_int64, P(P(abi.Tallies))]),
+        "trk_batch_fetch": (c_int, [c_void_p, P(P(abi.Tallies))]),
         "trk_batch_tune": (c_int, [c_void_p, c_uint64, c_int64]),
         "trk_batch_config": (c_int, [c_void_p, P(c_int32), P(c_int32), P(c_int64), P(c_int32), P(c_int32)]),
         "trk_batch_tune_log": (c_int, [c_void_p, P(c_int32), P(c_int32), abi.DP, c_int32, P(c_int32)]),
@@ -141,6 +142,14 @@ class Batch:
         if trace_cap:
             self.trace = trace[: min(ntr.value, trace_cap)]
             self.trace_total = ntr.value
+        return tal
+
+    def fetch(self, tallies=None):
+        """The tallies of the last run (one TallySet per impact), fetched from
+        the device; added into `tallies` when given."""
+        tal = tallies or self.new_tallies()
+        ptrs = (POINTER(abi.Tallies) * len(tal))(*[ctypes.pointer(t.s) for t in tal])
+        self._check(self.lib.trk_batch_fetch(self.h, ptrs), "trk_batch_fetch")
         return tal
 
     def tune(self, seed, first_iter=0):

@@ -9,8 +9,8 @@ in 1/A^3 scaled by 1e24 to 1/cm^3 (SO:728, 816, 1020), mean electron
 energy as a temperature Ee/ne * kB * 2/3 (SO:782-786), lattice energy as the
 sum over the time nodes so far (SO:931) and as a temperature per atom
 (SO:988).  Each table has one row per radius (Out_R) and one column per time
-node, as the files do; the text layout of the files (Intel list formats) is
-not reproduced.
+node, as the files do.  write_output() writes the files themselves with
+the reference's names, headers and Fortran fields.
 """
 import numpy as np
 
@@ -69,3 +69,145 @@ def radial_tables(norm, mat, vb=None):
             t[("holes_pot_energy", ia, js)] = norm["Out_Eh"][:, :, ia, js].T
             t[("holes_kin_energy", ia, js)] = norm["Out_Ehkin"][:, :, ia, js].T
     return t
+
+
+# ---------------------------------------------------------------- files
+
+def _e(x):
+    # the '(e)' field of Intel Fortran for REAL(8): E25.16
+    from .cache import fortran_e
+    return fortran_e(float(x), 25, 16)
+
+
+def _time_header(first, times, with_unit=True):
+    return first + "".join(f"{t:10.2f}" + ("[fs]   " if with_unit else "") for t in times) + " \n"
+
+
+def _matrix_file(path, first, times, labels, values, label_fmt="{:9.1f}"):
+    """Header of time nodes, then a row per label: the label and one value
+    per time node (values[row, node])."""
+    with open(path, "w") as f:
+        f.write(_time_header(first, times))
+        for i, lab in enumerate(labels):
+            f.write(label_fmt.format(lab) + "".join(_e(v) for v in values[i]) + " \n")
+
+
+def _shell_tag(atom, shell):
+    from .cache import ELEMENT_SYMBOL, shell_name
+    if shell.shl_num < 63:
+        return f"{ELEMENT_SYMBOL[atom.Z]}_{shell_name(shell.shl_num)}"
+    return shell_name(shell.shl_num)
+
+
+def write_output(folder, tallies, nmc, mat, time_grid, Out_R, include_photons, dos_E=None, hole_mass=None):
+    """Write the result files of Save_output (Sorting_output_data.f90:340-1140)
+    into folder, from a run's unnormalised tallies (divided by nmc here, as
+    MAIN:281-314 does): the angular distributions, Total_numbers.txt,
+    Hole_mean_diffusion_coefficient.txt, Total_energies.txt, the radial
+    electron / photon / lattice / track files, the energy spectra and every
+    shell's hole files, with the reference's names, headers and Fortran
+    fields ('(f10.2)' time headers, '(f9.1)' row labels, '(e)' values as
+    E25.16).  Quantities of the emission and field models, which this
+    library does not build, are written as zeros where the files hold them.
+    The parameter echo '!Parameters.txt' is not written.  Returns the file
+    names."""
+    import os
+    os.makedirs(folder, exist_ok=True)
+    n = normalise(tallies, nmc)
+    N = len(time_grid)
+    times = np.asarray(time_grid, dtype=float)
+    R = np.asarray(Out_R, dtype=float)
+    hole_mass = mat.hole_mass if hole_mass is None else hole_mass
+    written = []
+
+    def path(name):
+        written.append(name)
+        return os.path.join(folder, name)
+
+    # angular distributions (SO:536-591)
+    for name, key, first in (("Electrons_theta_distribution.txt", "Out_theta", "#Angle[deg] "),
+                             ("VB_holes_theta_distribution.txt", "Out_theta_h", "Angle[deg] ")):
+        th = n[key]
+        with open(path(name), "w") as f:
+            f.write(_time_header(first, times))
+            for i in range(180):
+                f.write(_e(float(i + 1)) + "".join(_e(th[k, i]) for k in range(N)) + " \n")
+    zero = np.zeros(N)
+    # totals (SO:618-640)
+    with open(path("Total_numbers.txt"), "w") as f:
+        if include_photons:
+            f.write("#Time[fs]    Ne    Ne_Emitted    Energy[eV]     Energy_Emitted[eV] N_photons\n")
+            for k in range(N):
+                f.write("".join(_e(v) for v in (times[k], n["Out_tot_Ne"][k], zero[k], n["Out_tot_E"][k], zero[k],
+                                                 n["Out_tot_Nphot"][k])) + "\n")
+        else:
+            f.write("#Time[fs]    Ne    Ne_Emitted    Energy[eV]     Energy_Emitted[eV]\n")
+            for k in range(N):
+                f.write("".join(_e(v) for v in (times[k], n["Out_tot_Ne"][k], zero[k], n["Out_tot_E"][k], zero[k]))
+                        + "\n")
+    # hole diffusion coefficient (SO:646-662): the file stays empty without hole transport
+    with open(path("Hole_mean_diffusion_coefficient.txt"), "w") as f:
+        if hole_mass < 1.0e3:
+            f.write("#Time[fs]    Diffusion_coeff[cm^2/s]\n")
+            for k in range(N):
+                f.write(_e(times[k]) + _e(n["Out_diff_coeff"][k]) + "\n")
+    # energies of electrons, atoms, field, photons and every shell's holes (SO:666-705)
+    with open(path("Total_energies.txt"), "w") as f:
+        f.write("#Time[fs]    Electrons[eV]   Atoms[eV]   Field[eV]  " + ("Photons[eV] " if include_photons else ""))
+        for atom in mat.atoms:
+            for sh in atom.shells:
+                f.write(_shell_tag(atom, sh) + "[eV]    ")
+        f.write(" \n")
+        Eh = n["Out_E_h"]
+        for k in range(N):
+            vals = [times[k], n["Out_E_e"][k], n["Out_E_at"][k], 0.0] + ([n["Out_E_phot"][k]] if include_photons else [])
+            for ia, atom in enumerate(mat.atoms):
+                for js in range(len(atom.shells)):
+                    vals.append(Eh[k, ia, js])
+            f.write("".join(_e(v) for v in vals) + " \n")
+    t = radial_tables(n, mat)
+    radial = ["Radial_electron_density[1_cm^-3].txt", "Radial_electron_energy[eV_A^-3].txt",
+              "Radial_electron_temperature[K].txt"]
+    if include_photons:
+        radial += ["Radial_photon_density[1_cm^-3].txt", "Radial_photon_energy[eV_A^-3].txt"]
+    for name in radial:
+        _matrix_file(path(name), "#Radius[A] ", times, R, t[name][:, :N])
+    # spectra (SO:852-906): the electron spectrum's rows are labelled by Out_R
+    ee = t["Electron_distribution_vs_E[1_eV].txt"]
+    with open(path("Electron_distribution_vs_E[1_eV].txt"), "w") as f:
+        f.write("#Energy[eV]   Spectrum[arb.units]\n")
+        f.write(_time_header("#time[fs]:  ", times, with_unit=False))
+        for i, r in enumerate(R):
+            f.write(f"{r:9.1f}" + "".join(_e(v) for v in ee[i, :N]) + " \n")
+    if dos_E is not None:
+        _matrix_file(path("VB_holes_distribution_vs_E[1_eV].txt"), "#Energy[eV] ", times, np.asarray(dos_E),
+                     t["VB_holes_distribution_vs_E[1_eV].txt"][:, :N])
+    for name in ("Radial_Lattice_energy[eV_A^-3].txt", "Radial_Track_energy[eV_A^-3].txt",
+                 "Radial_Lattice_temperature[K].txt"):
+        _matrix_file(path(name), "#Radius[A] ", times, R, t[name][:, :N])
+    # every shell's holes (SO:996-1138)
+    for ia, atom in enumerate(mat.atoms):
+        for js, sh in enumerate(atom.shells):
+            _matrix_file(path(f"Radial_{_shell_tag(atom, sh)}_holes_density[1_cm^-3].txt"), "#Radius[A] ", times, R,
+                         t[("holes_density", ia, js)][:, :N])
+    for ia, atom in enumerate(mat.atoms):
+        for js, sh in enumerate(atom.shells):
+            tag = _shell_tag(atom, sh)
+            if sh.shl_num < 63:
+                _matrix_file(path(f"Radial_{tag}_holes_energy[eV_A^-3].txt"), "#Radius[A] ", times, R,
+                             t[("holes_pot_energy", ia, js)][:, :N])
+            else:
+                _matrix_file(path(f"Radial_{tag}_holes_pot_energy[eV_A^-3].txt"), "#Radius[A] ", times, R,
+                             t[("holes_pot_energy", ia, js)][:, :N])
+                _matrix_file(path(f"Radial_{tag}_holes_kin_energy[eV_A^-3].txt"), "#Radius[A] ", times, R,
+                             t[("holes_kin_energy", ia, js)][:, :N])
+    for ia, atom in enumerate(mat.atoms):
+        for js, sh in enumerate(atom.shells):
+            if sh.shl_num == 63:
+                nh = n["Out_nh"][:, :, ia, js]
+                ek = n["Out_Ehkin"][:, :, ia, js]
+                with np.errstate(divide="ignore", invalid="ignore"):
+                    temp = np.where(nh > 0.0, ek / nh * G_KB * 2.0 / 3.0, 0.0)
+                _matrix_file(path(f"Radial_{_shell_tag(atom, sh)}_holes_temperature[K].txt"), "#Radius[A] ", times,
+                             R, temp.T[:, :N])
+    return written
