@@ -102,7 +102,8 @@ typedef struct trk_ion {
     double  E;                          /* [eV] total kinetic energy */
     int32_t Kind_Zeff;                  /* 0 Barkas, 1 Bohr, 2 Nikolaev-Dmitriev, 3 Schiwietz-Grande, 4 fixed */
     double  fixed_Zeff;
-    int32_t Kind_ion;                   /* 0 point charge (1, Brandt-Kitagawa, is not built) */
+    int32_t Kind_ion;                   /* 0 point charge, 1 Brandt-Kitagawa charge in the ion's mean free
+                                           paths (SHI_Total_IMFP, Cross_sections.f90:2431-2446) */
 } trk_ion;
 
 /* Numerical options (the Flag type, Objects.f90:135-180, and run controls). */

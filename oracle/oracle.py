@@ -67,6 +67,9 @@ def load():
         "or_auger_pick": (c_int, [c_void_p, c_double, c_double, c_int]),
         "or_new_angles": (None, [c_double, c_double, c_double, c_double, DP, DP]),
         "or_gap_walk": (c_double, [c_void_p, c_double, c_int]),
+        "or_bk_charge": (c_double, [c_double, c_double, c_double]),
+        "or_shi_zeff": (c_double, [c_void_p, c_double]),
+        "or_shi_diff_cs_bk": (c_double, [c_void_p, c_int, c_double, c_double, c_double, c_double]),
         "or_run_mc": (c_int, [c_void_p, c_int64, c_uint64, c_int64, DP, DP, c_int32, P(abi.Tallies), DP, c_int64,
                               P(c_int64)]),
         "or_run_mc_threads": (c_int, [c_void_p, c_int64, c_uint64, c_int64, DP, DP, c_int32, P(abi.Tallies), c_int]),

@@ -50,6 +50,11 @@ double or_loss_func(const or_ctx *c, int mode, double A, double E0, double G, do
 double or_diff_cs(const or_ctx *c, int shell, double Ele, double hw, double Mass);
 double or_diff_cs_phonon(const or_ctx *c, double Ele, double hw, double Mass);
 double or_shi_diff_cs(const or_ctx *c, int shell, double Ele, double MSHI, double Emax, double hw);
+/* Brandt-Kitagawa: the ion's effective charge at momentum hq, and the
+ * ion's differential inverse mean free path with it (tests) */
+double or_bk_charge(double hq, double Zp, double Zeff);
+double or_shi_zeff(const or_ctx *c, double E);   /* Equilibrium_charge_SHI at energy E */
+double or_shi_diff_cs_bk(const or_ctx *c, int shell, double Ele, double MSHI, double Emax, double hw);
 int    or_inel_point(const or_ctx *c, int kind, int shell, double Ele, double *L, double *dEdx,
                      double *hw, double *lam, int32_t cap, int32_t *n);
 int    or_grid(const trk_material *mat, double Emin, double Emax, double *out, int32_t cap, int32_t *n);

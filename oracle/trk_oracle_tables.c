@@ -650,6 +650,89 @@ double or_shi_zeff(const or_ctx *c, double E)
     }
 }
 
+/* Brand_Kitagawa, CS:2878-2889 */
+static double bk_charge(double hq, double Zp, double Zeff)
+{
+    double a = 0.2400519147;
+    double Z = (Zp - Zeff) / Zp;
+    double Z23 = (Z == 0.0) ? 0.0 : POW(Z, 2.0 / 3.0);
+    double kl = ((((hq * ((OR_A0 * 1.0e-10) * SQRT(OR_E))) * 2.0) * a) * Z23) / (POW(Zp, 2.0 / 3.0) * (1.0 - Z / 7.0));
+    double kl2 = kl * kl;
+    return (Zp * ((1.0 - Z) + kl2)) / (1.0 + kl2);
+}
+
+/* SHI_Diff_cross_section_BK, CS:2826-2875 */
+static double shi_diff_cs_bk(const or_ctx *c, int shell, double Ee, double MSHI, double Emax, double hw, double Zeff)
+{
+    double Zp = (double)c->ion.Zat;
+    double qmin = (hw / OR_H) / SQRT((2.0 * Ee) / MSHI);
+    double qmax = (SQRT(2.0 * OR_ME) / OR_H) * SQRT(Emax);
+    double dLs = 0.0, hq = qmin, dLs0 = 0.0;
+    while (hq < qmax) {
+        double dq = hq / 100.0;
+        double a = hq + dq / 2.0;
+        double rho = bk_charge(a, Zp, Zeff);
+        double temp1 = (or_imewq(c, hw, a, shell, 0) * rho) * rho;
+        double b = hq + dq;
+        rho = bk_charge(b, Zp, Zeff);
+        double dL = (or_imewq(c, hw, b, shell, 0) * rho) * rho;
+        dLs = dLs + ((dq / 6.0) * ((dLs0 + 4.0 * temp1) + dL)) / hq;
+        dLs0 = dL;
+        hq = hq + dq;
+    }
+    if (c->m.temp > 0.0) return dLs / (1.0 - EXP(-((hw / c->m.temp) * OR_KB)));
+    return dLs;
+}
+
+double or_shi_diff_cs_bk(const or_ctx *c, int shell, double Ee, double MSHI, double Emax, double hw)
+{
+    return shi_diff_cs_bk(c, shell, Ee, MSHI, Emax, hw, or_shi_zeff(c, Ee));
+}
+
+double or_bk_charge(double hq, double Zp, double Zeff) { return bk_charge(hq, Zp, Zeff); }
+
+/* SHI_TotIMFP_BK, CS:2748-2823: the ion's mean free path with the
+ * Brandt-Kitagawa charge inside the q integral (grid as SHI_TotIMFP; the
+ * stopping power sums with the left node of each step). */
+static void shi_tot_imfp_bk(const or_ctx *c, double Ele, int shell, double *Sigma, double *dEdx)
+{
+    double MSHI = OR_MP * c->ion.Mass;
+    double Zeff = or_shi_zeff(c, Ele);
+    double Egap = c->Egap;
+    double Emin = c->m.Ip[shell];
+    if (Emin <= 1.0e-3) Emin = 1.0e-3;
+    double Emax = (((4.0 * Ele) * OR_ME) * MSHI) / ((MSHI + OR_ME) * (MSHI + OR_ME));
+    if (c->o.plasmon_Emax) {
+        if (Emin == Egap) {
+            double Ep = SQRT((((((c->m.N_VB_el * c->m.At_Dens) * 1.0e6) * OR_H) * OR_H) / (OR_ME * OR_E0)) + (Egap * Egap));
+            if (Ep >= Emax) Emax = Ep;
+        }
+    }
+    const int n = 1000;
+    double lo_b, hi_b;
+    osc_window(c, shell, &lo_b, &hi_b);
+    double E_low = dmax(lo_b, Emin);
+    double E_high = dmin(hi_b, Emax);
+    double E = Emin, Ltot1 = 0.0, ddEdx = 0.0;
+    double Ltot0 = shi_diff_cs_bk(c, shell, Ele, MSHI, Emax, E, Zeff);
+    while (E <= Emax) {
+        double dE = or_define_dE(n, E, E_low, E_high, 1.0e-3);
+        double a = E + dE / 2.0;
+        double temp1 = shi_diff_cs_bk(c, shell, Ele, MSHI, Emax, a, Zeff);
+        double b = E + dE;
+        double dL = shi_diff_cs_bk(c, shell, Ele, MSHI, Emax, b, Zeff);
+        double temp2 = (dE / 6.0) * ((Ltot0 + 4.0 * temp1) + dL);
+        Ltot1 = Ltot1 + temp2;
+        ddEdx = ddEdx + E * temp2;
+        Ltot0 = dL;
+        E = E + dE;
+    }
+    double pre = (1.0 / ((OR_PI * OR_A0) * Ele)) * MSHI;
+    *Sigma = (pre / OR_ME) * Ltot1;
+    if (*Sigma > 1.0e30) *Sigma = 1.0e30;
+    *dEdx = (pre / OR_ME) * ddEdx;
+}
+
 /* SHI_TotIMFP, CS:2452-2597 (CDF branch).  If dn/dE/dL are given the
  * cumulative row dSedE (CS:2577-2582) is produced; its E entry is the left
  * node of each step, as in the reference. */
@@ -1130,7 +1213,8 @@ int or_inel_point(const or_ctx *c, int kind, int shell, double Ele, double *L, d
     else if (kind == TRK_TAB_H_EL) elastic_cs(c, Ele, 1, &S, &D, want ? &r : NULL);
     else if (kind == TRK_TAB_PHOT) tot_phot(c, Ele, shell, &S, &D);
     else if (kind == TRK_TAB_SHI) {
-        shi_tot_imfp(c, Ele, shell, &S, &D, NULL, NULL, NULL);
+        if (c->ion.Kind_ion == 1) shi_tot_imfp_bk(c, Ele, shell, &S, &D);
+        else shi_tot_imfp(c, Ele, shell, &S, &D, NULL, NULL, NULL);
         S = (S > 1.0e-10) ? 1.0 / S : 1.0e28;
     } else return TRK_ERR_ARG;
     *L = S;
@@ -1280,7 +1364,8 @@ int or_build_tables(or_ctx *c, int threads)
             if (!inside && i >= 2 && ts->E[i - 2] <= c->shi_hi && ts->E[i] >= c->shi_lo) inside = 1;
             if (!inside) { ts->L[s][i] = TRK_NAN; ts->dEdx[s][i] = TRK_NAN; continue; }
         }
-        shi_tot_imfp(c, ts->E[i], s, &S, &D, NULL, NULL, NULL);
+        if (c->ion.Kind_ion == 1) shi_tot_imfp_bk(c, ts->E[i], s, &S, &D);
+        else shi_tot_imfp(c, ts->E[i], s, &S, &D, NULL, NULL, NULL);
         ts->L[s][i] = (S > 1.0e-10) ? 1.0 / S : 1.0e28;
         ts->dEdx[s][i] = D;
     }

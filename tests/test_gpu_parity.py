@@ -23,13 +23,13 @@ SMALL = ["test", "test_metal", "test_sio2", "test_cdf", "test_polymer"]
 FULL = ["config1", "config2"]       # the Al-like/Xe and Al2O3-like/Au workloads at full size
 # the option branches the workloads above leave at their defaults: dispersion
 # kinds 2 and 3, ion charge models 1-4, a 300 K target, the plasmon integration
-# limit and logarithmic time grids (configs.OPTION_WORKLOADS)
-OPTIONS = ["opt_plasmon_pole", "opt_ritchie_warm", "opt_sg_warm", "opt_fixed_zeff"]
+# limit, logarithmic time grids and a Brandt-Kitagawa ion (configs.OPTION_WORKLOADS)
+OPTIONS = ["opt_plasmon_pole", "opt_ritchie_warm", "opt_sg_warm", "opt_fixed_zeff", "opt_bk_ion"]
 # table parity: test_sio2 has config5's tables (SiO2-like, U 20 MeV/u, phonon CDF, photons).
 # opt_ritchie_warm's full oracle tables take minutes (a deterministic pow per
 # q step); its rows are compared at sampled energies below instead.
 TABLES = ["test", "test_metal", "test_cdf", "test_polymer"] + FULL + ["config5"] + \
-    ["opt_plasmon_pole", "opt_sg_warm", "opt_fixed_zeff"]
+    ["opt_plasmon_pole", "opt_sg_warm", "opt_fixed_zeff", "opt_bk_ion"]
 
 
 @pytest.mark.parametrize("name", TABLES)
@@ -107,6 +107,35 @@ def test_mc_replay_bitwise(name):
     assert (g.n_events == c.n_events).all(), (g.n_events, c.n_events)
     ta, tb = sort_trace(g.trace), sort_trace(c.trace)
     mm = first_mismatch(ta, tb)
+    assert mm is None, mm
+    bad = compare_tallies(g, c)
+    assert not bad, bad
+
+
+def test_mc_replay_synthetic_photon_tables():
+    """The large single track's photon transport on synthetic attenuation
+    tables (configs.synthetic_photoabsorption through the EPDL branch of
+    Tot_Phot_IMFP, CS:855-877) instead of the CDF photon tables: the tables
+    set on the device read back as set, photons are absorbed, and every
+    event equals the oracle's replay on the same tables."""
+    from oracle import Oracle
+    lib = tg.load_library()
+    mat, ion, opt, run = tg.config_material("test_sio2")
+    mat.prepare(lib)
+    ctx = tg.Context(mat, ion, opt)
+    ctx.build_tables()
+    E, L = tg.configs.apply_photon_tables(ctx, tg.configs.synthetic_photoabsorption(mat))
+    for s in range(L.shape[0]):
+        assert np.array_equal(ctx.table(abi.TAB_PHOT, s)[1], L[s]), s
+    R, V = tg.radial_grid(run["NR"], run["Rmin"], run["Rmax"], mat.layer)
+    orc = Oracle(mat, ion, opt)
+    orc.inject_from(ctx)
+    g = ctx.run_mc(run["nmc"], run["seed"], R, V, trace_cap=3_000_000)
+    c = orc.run_mc(run["nmc"], run["seed"], R, V, tg.TallySet(orc.Nt, len(R), mat), trace_cap=3_000_000)
+    say(f"[photons] synthetic tables: events {g.n_events.tolist()}")
+    assert g.n_events[abi.EVENT_NAMES.index("photon")] > 0
+    assert (g.n_events == c.n_events).all(), (g.n_events, c.n_events)
+    mm = first_mismatch(sort_trace(g.trace), sort_trace(c.trace))
     assert mm is None, mm
     bad = compare_tallies(g, c)
     assert not bad, bad

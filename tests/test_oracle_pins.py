@@ -238,3 +238,61 @@ def test_gap_walk_brute_force(lib):
             assert o.lib.or_gap_walk(o.h, float(Eh), 1) == want_down, (trial, Eh)
             assert o.lib.or_gap_walk(o.h, float(Eh), 0) == want_up, (trial, Eh)
         o.close()
+
+
+def test_brandt_kitagawa_charge(lib):
+    """Brand_Kitagawa (CS:2878-2889): the ion's effective charge seen at
+    momentum transfer k is Zp (f + (k L)^2) / (1 + (k L)^2) with f = Zeff/Zp
+    the ionised fraction (Brandt and Kitagawa, Phys. Rev. B 25, 5631 (1982)):
+    Zeff at k -> 0, the bare nucleus Zp at k -> infinity.  The screening
+    length L read back from the charge is the reference's
+    0.48006 a0 (1 - f)^(2/3) / (Zp^(2/3) (1 - (1 - f)/7)) -- the published
+    length carries Zp^(1/3) where the reference has Zp^(-2/3), a factor Zp
+    (kept: the tables must be the reference's).  k is the physical momentum
+    hq sqrt(e) [1/m] of the reference's units."""
+    import oracle as orc
+    olib = orc.load()
+    a0_m = A0 * 1e-10
+    for Zp, Zeff in ((8.0, 3.0), (54.0, 20.5), (92.0, 60.0), (26.0, 26.0)):
+        assert olib.or_bk_charge(0.0, Zp, Zeff) == pytest.approx(Zeff, rel=1e-15)
+        assert olib.or_bk_charge(1e40, Zp, Zeff) == pytest.approx(Zp, rel=1e-12)
+        f = Zeff / Zp
+        L_ref = 2 * 0.2400519147 * a0_m * (1 - f) ** (2 / 3) / (Zp ** (2 / 3) * (1 - (1 - f) / 7))
+        prev = Zeff
+        for hq in np.geomspace(1e3, 1e12, 40):
+            rho = olib.or_bk_charge(float(hq), Zp, Zeff)
+            assert prev <= rho * (1 + 1e-15) <= Zp * (1 + 1e-15)
+            prev = rho
+            if Zeff < Zp and Zeff < rho < Zp * (1 - 1e-9):
+                k = hq * math.sqrt(EV)
+                kl2 = (rho / Zp - f) / (1 - rho / Zp)
+                assert math.sqrt(kl2) / k == pytest.approx(L_ref, rel=1e-7), (Zp, Zeff, hq)
+
+
+def test_brandt_kitagawa_cross_section_brackets(lib):
+    """SHI_Diff_cross_section_BK (CS:2826-2875) is the point-charge q
+    integral (SHI_Diff_cross_section, pinned by quadrature elsewhere) with
+    the squared charge inside: a fully stripped ion (Zeff = Zp, charge Zp at
+    every q) gives exactly Zp^2 times the point-charge value, a partly
+    stripped one a value between Zeff^2 and Zp^2 times it."""
+    mat, ion, opt, run = tg.config_material("test")
+    mat.prepare(lib)
+    MSHI = 1.67262158e-27 * ion.Mass
+    Emax = 4.0 * ion.E * 9.10938215e-31 * MSHI / (MSHI + 9.10938215e-31) ** 2
+    for kz, fz in ((4, float(ion.Zat)), (1, 0.0)):
+        ion.Kind_Zeff, ion.fixed_Zeff, ion.Kind_ion = kz, fz, 1
+        o = Oracle(mat, ion, opt)
+        Zp = float(ion.Zat)
+        for s in range(sum(mat.n_shells)):
+            for hw in (30.0, 120.0, 600.0):
+                p = o.lib.or_shi_diff_cs(o.h, s, ion.E, MSHI, Emax, hw)
+                b = o.lib.or_shi_diff_cs_bk(o.h, s, ion.E, MSHI, Emax, hw)
+                if p <= 0.0:
+                    continue
+                if kz == 4:
+                    assert b == pytest.approx(Zp * Zp * p, rel=1e-12), (s, hw)
+                else:
+                    Zeff = o.lib.or_shi_zeff(o.h, ion.E)
+                    assert 0.0 < Zeff < Zp
+                    assert Zeff * Zeff * p < b < Zp * Zp * p, (s, hw, Zeff)
+        o.close()

@@ -2,6 +2,7 @@
 (Radius_for_distributions, Sorting_output_data.f90:1363-1435) and the values
 MAIN and Save_output write after it (trekis_gpu.output)."""
 import numpy as np
+import pytest
 
 from trekis_gpu import configs, inputs
 
@@ -127,3 +128,25 @@ def test_output_files_follow_save_output(tmp_path):
     output.write_output(str(tmp_path / "heavy"), t, 4, mat, times, R, include_photons=False, hole_mass=1e30)
     assert (tmp_path / "heavy" / "Hole_mean_diffusion_coefficient.txt").read_text() == ""
     assert not (tmp_path / "heavy" / "Radial_photon_density[1_cm^-3].txt").exists()
+
+
+def test_photon_paths_from_cross_sections():
+    """EPDL branch of Tot_Phot_IMFP (CS:855-877): a core shell's photon path
+    is 1/(n x_a sigma), the valence band's 1/(n sigma_VB); no absorption
+    below a shell's edge gives 1e30."""
+    mat = configs._sio2_like(layer=100.0)
+    E = np.geomspace(1.0, 1.0e5, 50)
+    sig = configs.synthetic_photoabsorption(mat)
+    L = configs.photon_paths_from_cross_sections(mat, E, sig)
+    n = mat.At_Dens * 1e-24
+    shells = [(ia, js, sh) for ia, a in enumerate(mat.atoms) for js, sh in enumerate(a.shells)]
+    assert L.shape == (len(shells), len(E))
+    for s, (ia, js, sh) in enumerate(shells):
+        x = 1.0 if (ia == 0 and js == len(mat.atoms[0].shells) - 1) else mat.atoms[ia].pers / mat.sum_pers
+        above = E >= sh.Ip
+        assert (L[s][~above] == 1e30).all()
+        assert np.allclose(L[s][above] * n * x * sig(s, E[above]), 1.0, rtol=1e-14)
+        # the E^-3 fall above the edge: the path grows as E^3
+        if above.sum() > 2:
+            e1, e2 = E[above][0], E[above][-1]
+            assert L[s][above][-1] / L[s][above][0] == pytest.approx((e2 / e1) ** 3, rel=1e-12)

@@ -57,7 +57,7 @@ static int validate(const trk_material *m, const trk_ion *ion, const trk_options
     if (m->work_function > 0.0) { why = "surface emission (work_function > 0) is not built"; return 1; }
     if (!(m->At_Dens > 0.0) || !(m->Layer > 0.0)) { why = "At_Dens and Layer must be positive"; return 1; }
     if (ion->Zat < 1 || !(ion->Mass > 0.0) || !(ion->E > 0.0)) { why = "invalid ion"; return 1; }
-    if (ion->Kind_ion != 0) { why = "Brandt-Kitagawa ions are not built"; return 1; }
+    if (ion->Kind_ion != 0 && ion->Kind_ion != 1) { why = "Kind_ion must be 0 (point charge) or 1 (Brandt-Kitagawa)"; return 1; }
     if (!(o->Tim > 0.0) || !(o->dt > 0.0)) { why = "Tim and dt must be positive"; return 1; }
     if (o->dt_flag >= 1 && o->dt < 2.0) { why = "a logarithmic time grid needs dt >= 2 (RF:299)"; return 1; }
     // kind 4 is the Delta-CDF model in the reference's TotIMFP (CS:955), which is not built

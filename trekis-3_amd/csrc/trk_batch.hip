@@ -481,7 +481,8 @@ int trk_ctx_create_ion(const trk_ctx *mat, const trk_ion *ion, trk_ctx **out)
     if (!mat->tables_ready) return set_error(c, TRK_ERR_STATE, "the material context's tables must be built first");
     if (mat->base) return set_error(c, TRK_ERR_ARG, "the material context is itself an ion context");
     if (ion->Zat < 1 || !(ion->Mass > 0.0) || !(ion->E > 0.0)) return set_error(c, TRK_ERR_ARG, "invalid ion");
-    if (ion->Kind_ion != 0) return set_error(c, TRK_ERR_ARG, "Brandt-Kitagawa ions are not built");
+    if (ion->Kind_ion != 0 && ion->Kind_ion != 1)
+        return set_error(c, TRK_ERR_ARG, "Kind_ion must be 0 (point charge) or 1 (Brandt-Kitagawa)");
     c->base = mat;
     c->hm = mat->hm;
     c->hm.ion = *ion;

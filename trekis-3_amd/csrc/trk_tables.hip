@@ -186,15 +186,55 @@ TRK_DEV double shi_diff_cs(const MatDev &m, int shell, double Ee, double MSHI, d
     return dLs * T_fact;
 }
 
+// Brand_Kitagawa, Cross_sections.f90:2878-2889: the effective charge of an
+// ion of nuclear charge Zp and equilibrium charge Zeff seen at momentum
+// transfer hq (reference units: [sqrt(eV/J)/m], times a0 sqrt(e) in metres)
+TRK_DEV double bk_charge(double hq, double Zp, double Zeff)
+{
+    const double a = 0.2400519147;
+    const double Z = (Zp - Zeff) / Zp;
+    const double Z23 = (Z == 0.0) ? 0.0 : trk_pow(Z, 2.0 / 3.0);   // 0**(2/3) = 0
+    const double kl = ((((hq * ((kA0 * 1.0e-10) * trk_sqrt(kE))) * 2.0) * a) * Z23) /
+                      (trk_pow(Zp, 2.0 / 3.0) * (1.0 - Z / 7.0));
+    const double kl2 = kl * kl;
+    return (Zp * ((1.0 - Z) + kl2)) / (1.0 + kl2);
+}
+
+// SHI_Diff_cross_section_BK, Cross_sections.f90:2826-2875: the point-charge
+// integrand weighted by the squared Brandt-Kitagawa charge at each q
+TRK_DEV double shi_diff_cs_bk(const MatDev &m, int shell, double Ee, double MSHI, double Emax, double hw, double Zp,
+                              double Zeff)
+{
+    const double qmin = (hw / kH) / trk_sqrt((2.0 * Ee) / MSHI);
+    const double qmax = (trk_sqrt(2.0 * kMe) / kH) * trk_sqrt(Emax);
+    double dLs = 0.0, hq = qmin, dLs0 = 0.0;
+    for (int g = 0; hq < qmax && g < kMaxQSteps; ++g) {
+        double dq = hq / 100.0;
+        double a = hq + dq / 2.0;
+        double rho = bk_charge(a, Zp, Zeff);
+        double temp1 = (imewq(m, hw, a, shell, 0) * rho) * rho;
+        double b = hq + dq;
+        rho = bk_charge(b, Zp, Zeff);
+        double dL = (imewq(m, hw, b, shell, 0) * rho) * rho;
+        dLs = dLs + ((dq / 6.0) * ((dLs0 + 4.0 * temp1) + dL)) / hq;
+        dLs0 = dL;
+        hq = hq + dq;
+    }
+    // dLs / (1 - exp(-hw/T kB)); at T = 0 the exponential is 0
+    if (m.temp > 0.0) return dLs / (1.0 - trk_exp(-((hw / m.temp) * kKb)));
+    return dLs;
+}
+
 // ---------------------------------------------------------------- row jobs
 enum RowType { ROW_INEL = 0, ROW_PHONON = 1, ROW_SHI = 2, ROW_SHI_DIFF = 3 };
 
 struct RowJob {
     double Ele, Estart, Emax, E_low, E_high, dE_min, Mass, Zeff, MSHI;
+    double Zp;   // the ion's nuclear charge (ROW_SHI with bk)
     int n, shell, type, with_row;
     int kind, ie;
     int q32;     // FP32 compensated quadrature (ROW_INEL, ROW_SHI, ROW_SHI_DIFF)
-    int pad_;
+    int bk;      // ROW_SHI of a Brandt-Kitagawa ion (SHI_TotIMFP_BK, CS:2748-2823)
 };
 
 // ---------------------------------------------------------------- FP32 quadrature
@@ -368,7 +408,9 @@ TRK_DEV double eval_point(const MatDev &m, const RowJob &j, double x)
     switch (j.type) {
     case ROW_INEL: return diff_cs(m, j.shell, j.Ele, x, j.Mass);
     case ROW_PHONON: return diff_cs_phonon(m, j.Ele, x, j.Mass);
-    default: return shi_diff_cs(m, j.shell, j.Ele, j.MSHI, j.Emax, x);
+    default:
+        if (j.bk) return shi_diff_cs_bk(m, j.shell, j.Ele, j.MSHI, j.Emax, x, j.Zp, j.Zeff);
+        return shi_diff_cs(m, j.shell, j.Ele, j.MSHI, j.Emax, x);
     }
 }
 
@@ -499,7 +541,7 @@ __global__ void k_row_scan(const RowJob *__restrict__ jobs, int nrows, const int
         if (j.type == ROW_PHONON) temp2 = (trk_fabs(dE) / 6.0) * ((Ltot0 + 4.0 * temp1) + dL);
         else temp2 = (dE / 6.0) * ((Ltot0 + 4.0 * temp1) + dL);
         Ltot1 = Ltot1 + temp2;
-        if (j.type == ROW_SHI || j.type == ROW_SHI_DIFF) {
+        if ((j.type == ROW_SHI && !j.bk) || j.type == ROW_SHI_DIFF) {
             double a = E + dE / 2.0;
             double b = E + dE;
             ddEdx = ddEdx + (dE / 6.0) * (((E * Ltot0) + ((a * 4.0) * temp1)) + (b * dL));
@@ -540,10 +582,11 @@ __global__ void k_row_scan(const RowJob *__restrict__ jobs, int nrows, const int
             for (int64_t t = 0; t < rlen; ++t) cum[t] = (cum[t] < 1.0e-15) ? 1.0e20 : 1.0 / (Z2M * cum[t]);
     } else if (j.type == ROW_SHI) {
         double pre = (1.0 / ((kPi * kA0) * j.Ele)) * j.MSHI;
-        double Sigma = (((pre / kMe) * j.Zeff) * j.Zeff) * Ltot1;
+        // a Brandt-Kitagawa ion carries its charge inside the integrand
+        double Sigma = j.bk ? (pre / kMe) * Ltot1 : (((pre / kMe) * j.Zeff) * j.Zeff) * Ltot1;
         if (Sigma > 1.0e30) Sigma = 1.0e30;
         out.L[tix] = (Sigma > 1.0e-10) ? 1.0 / Sigma : 1.0e28;
-        out.dEdx[tix] = (((pre / kMe) * j.Zeff) * j.Zeff) * ddEdx;
+        out.dEdx[tix] = j.bk ? (pre / kMe) * ddEdx : (((pre / kMe) * j.Zeff) * j.Zeff) * ddEdx;
     }
 }
 
@@ -752,6 +795,10 @@ static RowJob shi_job(const HostMat &h, double Ele, int s, int diff, int ie)
     j.shell = s;
     j.type = diff ? ROW_SHI_DIFF : ROW_SHI;
     j.with_row = diff;
+    // the ion's mean free paths of a Brandt-Kitagawa ion (SHI_Total_IMFP,
+    // CS:2431-2446); its transfer rows stay those of a point charge
+    j.bk = (!diff && h.ion.Kind_ion == 1) ? 1 : 0;
+    j.Zp = (double)h.ion.Zat;
     j.kind = diff ? TRK_TAB_SHI_DIFF : TRK_TAB_SHI;
     j.ie = ie;
     return j;
@@ -890,7 +937,7 @@ int build_tables(const HostMat &h, const MatDev &md, TableSet &ts, hipStream_t s
     // phonon rows keep the reference's)
     if (h.o.quadrature == 1)
         for (RowJob &j : jobs)
-            if (j.type != ROW_PHONON) j.q32 = 1;
+            if (j.type != ROW_PHONON && !j.bk) j.q32 = 1;
     if (own) {
         T[TRK_TAB_E_INEL].has_rows.assign(nsh, 1);
         T[TRK_TAB_H_INEL].has_rows[h.vb] = 1;

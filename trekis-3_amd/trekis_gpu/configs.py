@@ -326,7 +326,10 @@ def config_material(name):
     config1  Al-like metal, Xe 1 MeV/u, 100 iterations, 0-100 fs (dt 0.1)
     config2  Al2O3-like, Au 11 MeV/u, 10^4 iterations, 0-1000 fs (dt 1)
     config5  SiO2-like with phonon quasi-elastic scattering and photons, U 20
-             MeV/u over a 1 um track, 10^6 iterations, 0-1000 fs, 200 radial bins
+             MeV/u over a 1 um track, 10^6 iterations, 0-1000 fs, 200 radial bins;
+             its synthetic photon attenuation tables replace the built photon
+             tables after the build (apply_photon_tables with
+             synthetic_photoabsorption)
     test     config2's target with a reduced electron grid (2 keV), O ion at
              0.5 MeV/u, 0-100 fs: small enough for the CPU oracle
     test_sio2  config5 on a 100 A track segment, 2 iterations (the full
@@ -402,10 +405,11 @@ def config_material(name):
     elif name in OPTION_WORKLOADS:
         # the small workloads with the reference's option branches that the
         # configurations above leave at their defaults switched on
-        base, dr, kz, fz, temp, plasmon, dt_flag, dt = OPTION_WORKLOADS[name]
+        base, dr, kz, fz, temp, plasmon, dt_flag, dt, *rest = OPTION_WORKLOADS[name]
         mat, ion, opt, run = config_material(base)
         mat.temp = temp
         ion.Kind_Zeff, ion.fixed_Zeff = kz, fz
+        ion.Kind_ion = rest[0] if rest else 0
         opt.kind_of_DR, opt.plasmon_Emax = dr, plasmon
         opt.dt_flag, opt.dt = dt_flag, dt
         return mat, ion, opt, run
@@ -429,11 +433,15 @@ def config_material(name):
 #   opt_fixed_zeff    a fixed ion charge of 3.5, a 300 K metal (Bose factor of
 #                     the phonon CDF, CS:3290-3295), logarithmic time grid of
 #                     ratio 10 (the reference's own INPUT_PARAMETERS.txt:6 setting)
+#   opt_bk_ion        a Brandt-Kitagawa ion (Kind_ion = 1, a 9th entry): its
+#                     mean free paths with the charge form factor inside the q
+#                     integral (SHI_TotIMFP_BK, CS:2748-2889), Bohr charge
 OPTION_WORKLOADS = {
     "opt_plasmon_pole": ("test", 2, 1, 8.0, 0.0, 1, 1, 2.0),
     "opt_ritchie_warm": ("test", 3, 2, 8.0, 300.0, 0, 0, 1.0),
     "opt_sg_warm": ("test", 1, 3, 8.0, 300.0, 0, 0, 1.0),
     "opt_fixed_zeff": ("test_metal", 1, 4, 3.5, 300.0, 0, 1, 10.0),
+    "opt_bk_ion": ("test", 1, 1, 8.0, 0.0, 0, 0, 1.0, 1),
 }
 
 
@@ -508,3 +516,54 @@ def config4_batch(n_impacts=None, precision=1):
         ions.append((ion, im["material"]))
     run = dict(NR=100, Rmin=1.0, Rmax=1000.0, seed=0x5EED0004)
     return mats, opts, ions, run
+
+
+# -------------------------------------------------------------- photon attenuation tables
+def photon_paths_from_cross_sections(desc, E, sigma):
+    """Per-shell photon mean free paths [A] on the photon grid E from
+    per-shell photoabsorption cross sections [A^2 per atom], as the EPDL
+    branch of Tot_Phot_IMFP converts them (Cross_sections.f90:855-877): a
+    core shell of atom a, L = 1 / (n x_a sigma) with n the atomic density
+    [1/A^3] (At_Dens 1e-24) and x_a = Pers_a / sum Pers; the valence band
+    (the last shell of the first atom), L = 1 / (n sigma_VB) with sigma_VB
+    the stoichiometry-weighted cross section of the shells that form it,
+    given as it is.  sigma: callable (flat shell index, E array) -> array;
+    a zero cross section gives 1e30 (no absorption)."""
+    n = desc.At_Dens * 1.0e-24
+    xs = np.array([a.pers for a in desc.atoms]) / desc.sum_pers
+    out = []
+    s = 0
+    vb = len(desc.atoms[0].shells) - 1
+    for ia, atom in enumerate(desc.atoms):
+        for js in range(len(atom.shells)):
+            sig = np.asarray(sigma(s, np.asarray(E)), dtype=float)
+            w = n if (ia == 0 and js == vb) else n * xs[ia]
+            with np.errstate(divide="ignore"):
+                L = np.where(sig > 0.0, 1.0 / (w * sig), 1.0e30)
+            out.append(L)
+            s += 1
+    return np.array(out)
+
+
+def synthetic_photoabsorption(desc, sigma_edge=2.0e-3):
+    """Synthetic photoabsorption cross sections of the 'large single track'
+    configuration (no EPDL data here): per shell, zero below its ionisation
+    potential and sigma_edge x (Nel / 2) x (Ip / E)^3 above it [A^2], the
+    E^-3 fall of photoabsorption above an edge."""
+    shells = [sh for a in desc.atoms for sh in a.shells]
+
+    def sigma(s, E):
+        sh = shells[s]
+        return np.where(E >= sh.Ip, sigma_edge * (sh.Nel / 2.0) * (sh.Ip / np.maximum(E, 1e-30)) ** 3, 0.0)
+    return sigma
+
+
+def apply_photon_tables(ctx, sigma):
+    """Replace a built context's photon tables (trk_table_set, kind PHOT) by
+    those of the cross sections sigma and re-derive its fused totals."""
+    E, _, _ = ctx.table(abi.TAB_PHOT, 0)
+    L = photon_paths_from_cross_sections(ctx.desc, E, sigma)
+    for s in range(L.shape[0]):
+        ctx.set_table(abi.TAB_PHOT, s, L=L[s])
+    ctx.finish_tables()
+    return E, L
