@@ -144,6 +144,9 @@ def run_leg(tg, abi, args, precision, first_iter, quadrature=None):
     ctx = tg.Context(mat, ion, opt)
     t0 = time.perf_counter()
     ctx.build_tables()
+    if args.config == "config5":
+        # the large single track's synthetic photon attenuation tables
+        tg.configs.apply_photon_tables(ctx, tg.configs.synthetic_photoabsorption(mat))
     t_tables = time.perf_counter() - t0
     name = "mixed" if precision else "fp64"
     print(f"[bench] {name}: tables built in {t_tables:.2f} s", file=sys.stderr, flush=True)
@@ -301,7 +304,8 @@ def main():
     ap.add_argument("--cpu-iters", type=int, default=2,
                     help="oracle iterations per host thread for the CPU baseline (0 = skip)")
     ap.add_argument("--config", default="config2",
-                    help="config2: Al2O3-like / Au 11 MeV/u; config4: the batch of ion impacts on 4 materials")
+                    help="config2: Al2O3-like / Au 11 MeV/u; config4: the batch of ion impacts on 4 materials; "
+                         "config5: the large single track (SiO2-like / U 20 MeV/u, 1 um, photons, phonons)")
     ap.add_argument("--impacts", type=int, default=256, help="config4: impacts per batch")
     ap.add_argument("--batch-capacity", type=int, default=1 << 22, help="config4: list capacity per kind and group")
     ap.add_argument("--tune", type=int, default=1, help="config4: run the autotuner first (1) or not (0)")
@@ -444,8 +448,11 @@ def main():
         "vs_baseline": None,
         "dtype": "f32 sampling / f64 state and tallies" if mixed else "f64",
         "data": "synthetic",
-        "config": {"workload": f"{mat.name} target, Au 11 MeV/u ion, 0-1000 fs x 100 radial bins, "
-                               f"{B} iterations per step"
+        "config": {"workload": (f"{mat.name} target, Au 11 MeV/u ion, 0-1000 fs x 100 radial bins, "
+                                if args.config != "config5" else
+                                f"{mat.name} target, U 20 MeV/u ion over a 1 um track, phonon quasi-elastic scattering, "
+                                f"photons on synthetic attenuation tables, 0-1000 fs x 200 radial bins, ")
+                               + f"{B} iterations per step"
                                + (", mixed leg on FP32-quadrature tables" if mixed and args.quadrature else ""),
                    "precision": args.precision, "iters_per_step": B,
                    "table_quadrature": "fp32 compensated" if mixed and args.quadrature else "reference fp64",
