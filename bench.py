@@ -237,8 +237,53 @@ def run_batch_leg(tg, args, precision, n_impacts):
         print(f"[bench] batch {name}: step {k}: {int(e.sum())} events, {ms:.1f} ms on the device, {r} rounds",
               file=sys.stderr, flush=True)
     return {"batch": b, "ev": ev, "t_total": t_total, "fetch_s": fetch_s, "dev_ms": dev_ms, "rounds": rounds, "tune": tune,
+            "R": R, "seed": run["seed"],
             "tune_log": b.tune_log() if tune else None, "config": b.config(), "t_tables": t_tables, "tals": tals,
             "mats": mats, "ions": ions, "V": V, "impacts": imps, "mctx": mctx}
+
+
+def batch_cpu_baseline(tg, args, leg):
+    """The oracle on a bounded sample of the batch: the first impacts, one
+    per host thread (impact i is iteration i of its own context, as on the
+    GPU), each on that impact's GPU-built tables, in FP64, side by side."""
+    from concurrent.futures import ThreadPoolExecutor
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    from oracle import Oracle
+    avail = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    threads = max(1, int(os.environ.get("OMP_NUM_THREADS", avail)))
+    imps = leg["impacts"]
+    sample = list(range(min(len(imps), threads)))
+    R, V = leg["R"], leg["V"]
+    orcs = []
+    for i in sample:
+        c = imps[i]
+        o = Oracle(c.desc, c.ion, c.opt)
+        o.inject_from(c)
+        orcs.append(o)
+
+    def one(j):
+        t = tg.TallySet(orcs[j].Nt, len(R), imps[sample[j]].desc)
+        t0 = time.perf_counter()
+        orcs[j].run_mc(1, leg["seed"], R, V, t, first_iter=sample[j])
+        return int(t.n_events.sum()), time.perf_counter() - t0
+
+    ts = time.perf_counter()
+    with ThreadPoolExecutor(max_workers=threads) as ex:
+        res = list(ex.map(one, range(len(sample))))
+    dt = time.perf_counter() - ts
+    for o in orcs:
+        o.close()
+    evs = sum(e for e, _ in res)
+    busy = sum(b for _, b in res)
+    # histories differ ~100x in length, so a few impacts per thread leave
+    # threads idle; the value is the all-core rate of a balanced schedule
+    # (every thread busy, as the reference's dynamic OpenMP loop approaches
+    # on a full batch): threads x events / summed per-history seconds
+    return {"value": threads * evs / busy, "unit": "events/s", "cores": threads, "kind": "port",
+            "sample": f"impacts 0-{len(sample) - 1} of the batch (one history each, one per host thread), FP64, "
+                      f"on the GPU-built tables: {evs} events, {busy:.1f} s of summed per-history time; "
+                      f"wall {dt:.1f} s ({evs / dt:.4g} events/s as run, threads idle after their history)",
+            "single_core_value": evs / busy}
 
 
 def main_batch(args):
@@ -269,6 +314,7 @@ def main_batch(args):
                                              "tallies": {k: v for k, v in pm.items() if k in RADIAL}}
             compare["max_radial_tally_rel_error"] = worst
             compare["per_material"] = per
+    cpu = batch_cpu_baseline(tg, args, leg) if args.cpu_iters > 0 else None
     line = {
         "metric": "collision events/s on 1 GPU, batched impacts, mixed vs FP64" if mixed
                   else "collision events/s on 1 GPU, batched impacts (FP64 path)",
@@ -287,6 +333,7 @@ def main_batch(args):
                                                    if args.batch_tallies else None),
                    "launch": leg["config"], "autotuner": {"choice": leg["tune"], "timings": leg["tune_log"]},
                    "table_build_s": leg["t_tables"], "vs_fp64": compare},
+        "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)
 

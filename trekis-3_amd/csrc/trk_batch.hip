@@ -558,9 +558,8 @@ int trk_batch_create(trk_ctx *const *impacts, int32_t n, const double *Out_R, co
     for (int i = 0; i < n; ++i) {
         trk_ctx *c = impacts[i];
         ions[i] = ion_view_host(c);
-        if (b->mixed && !c->ts.sd_consts) {
-            mx::k_shi_consts<<<(c->hm.nsh + 63) / 64, 64, 0, b->stream>>>(ions[i], c->md.Ip, c->hm.nsh, c->ts.sd_M, c->ts.sd_c);
-            BHIP(hipGetLastError());
+        if (!c->ts.sd_consts) {
+            BHIP(launch_shi_consts(c, ions[i], b->mixed, b->stream));
             c->ts.sd_consts = true;
         }
     }

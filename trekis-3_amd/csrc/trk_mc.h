@@ -49,7 +49,7 @@ struct IonDev {
     int Z, kind_zeff, nS, pad_;
     const double *ES, *LSsh, *shi_path, *lnES, *ln_LSsh, *ln_shi_path;
     const int *sd_n; const int64_t *sd_off; const double *sd_E, *sd_L, *sd_inv; const int *sd_nonmono;
-    const int *sd_M; const double *sd_c;
+    const int *sd_M; const double *sd_c; const int *sd_bk;
 };
 
 // Round state of one material group of a batch (device memory): particles
@@ -118,7 +118,7 @@ struct McDev {
     // fields above stay those the FP64 kernel was tuned with: per-shell
     // constants of shi_transfer (k_shi_consts), logs of the SHI grid, of
     // its per-shell paths and of the fused path
-    const int *sd_M; const double *sd_c;
+    const int *sd_M; const double *sd_c; const int *sd_bk;
     const double *lnES, *ln_LSsh, *ln_shi_path;
     // batch runs only: the impacts' ions (indexed by iteration - iter0), the
     // events a particle may take per round, this group's round state
@@ -169,6 +169,7 @@ constexpr int kTransportBlockMx = 768;
     template <bool kAll> __global__ void k_transport(McDev din);             \
     __global__ void k_ion_init(McDev d, int nb, uint32_t first_iter);        \
     __global__ void k_shi_consts(IonDev iv, const double *Ip, int nsh, int *sd_M, double *sd_c); \
+    __global__ void k_shi_buckets(IonDev iv, int nsh, const int *sd_M, const double *sd_c, int *sd_bk); \
     template <bool kAll> __global__ void k_batch(BatchCtl *ctl, int n_groups); \
     __global__ void k_ion_init_batch(int g, int nb, const int *idx);        \
     hipError_t batch_upload(const McDev *mc, int n_groups, hipStream_t st);
@@ -178,6 +179,7 @@ namespace mx { TRK_MC_KERNELS }
 
 // The ion view of a context's own ion (host side).
 IonDev ion_view_host(const trk_ctx *c);
+hipError_t launch_shi_consts(const trk_ctx *c, const IonDev &iv, bool mixed, hipStream_t st);
 // Transport descriptor of a context's tables (trk_mc.hip).
 void fill_mcdev(const trk_ctx *c, McDev &d);
 // LDS layout of the staged tables (trk_mc.hip); true when all are staged.

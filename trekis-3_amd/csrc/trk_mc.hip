@@ -385,7 +385,7 @@ void fill_mcdev(const trk_ctx *c, McDev &d)
     d.nS = T[TRK_TAB_SHI].n; d.ES = T[TRK_TAB_SHI].E; d.LSsh = T[TRK_TAB_SHI].L; d.shi_path = ts.shi_path;
     const TableDev &td = T[TRK_TAB_SHI_DIFF];
     d.sd_n = ts.sd_n; d.sd_off = ts.sd_off; d.sd_E = td.dE; d.sd_L = td.dL; d.sd_inv = td.dInv;
-    d.sd_nonmono = td.dNonMono; d.sd_M = ts.sd_M; d.sd_c = ts.sd_c;
+    d.sd_nonmono = td.dNonMono; d.sd_M = ts.sd_M; d.sd_c = ts.sd_c; d.sd_bk = ts.sd_bk;
     d.lnE = ts.ln_E[TRK_TAB_E_INEL]; d.ln_el_imfp = ts.ln_el_imfp;
     d.lnEel = ts.ln_E[TRK_TAB_E_EL]; d.ln_Lel = ts.ln_Lel;
     d.lnEH = ts.ln_E[TRK_TAB_H_INEL]; d.ln_hole_imfp = ts.ln_hole_imfp;
@@ -526,6 +526,20 @@ bool plan_lds(McDev &d, int Nt, int NR, int n_dos, int nsh, size_t limit, size_t
     return all;
 }
 
+// k_shi_consts then k_shi_buckets of the path's precision on a table set.
+hipError_t launch_shi_consts(const trk_ctx *c, const IonDev &iv, bool mixed, hipStream_t st)
+{
+    const int nsh = c->hm.nsh;
+    if (mixed) {
+        mx::k_shi_consts<<<(nsh + 63) / 64, 64, 0, st>>>(iv, c->md.Ip, nsh, c->ts.sd_M, c->ts.sd_c);
+        mx::k_shi_buckets<<<nsh, 128, 0, st>>>(iv, nsh, c->ts.sd_M, c->ts.sd_c, c->ts.sd_bk);
+    } else {
+        fp64::k_shi_consts<<<(nsh + 63) / 64, 64, 0, st>>>(iv, c->md.Ip, nsh, c->ts.sd_M, c->ts.sd_c);
+        fp64::k_shi_buckets<<<nsh, 128, 0, st>>>(iv, nsh, c->ts.sd_M, c->ts.sd_c, c->ts.sd_bk);
+    }
+    return hipGetLastError();
+}
+
 // The ion of a context as the event code reads it.
 IonDev ion_view_host(const trk_ctx *c)
 {
@@ -537,7 +551,7 @@ IonDev ion_view_host(const trk_ctx *c)
     v.ES = ts.tab[TRK_TAB_SHI].E; v.LSsh = ts.tab[TRK_TAB_SHI].L; v.shi_path = ts.shi_path;
     v.lnES = ts.ln_E[TRK_TAB_SHI]; v.ln_LSsh = ts.ln_L[TRK_TAB_SHI]; v.ln_shi_path = ts.ln_shi_path;
     v.sd_n = ts.sd_n; v.sd_off = ts.sd_off; v.sd_E = td.dE; v.sd_L = td.dL; v.sd_inv = td.dInv;
-    v.sd_nonmono = td.dNonMono; v.sd_M = ts.sd_M; v.sd_c = ts.sd_c;
+    v.sd_nonmono = td.dNonMono; v.sd_M = ts.sd_M; v.sd_c = ts.sd_c; v.sd_bk = ts.sd_bk;
     return v;
 }
 
@@ -584,12 +598,11 @@ static int run_core(trk_ctx *c, McState *s, int64_t nmc, uint64_t seed, int64_t 
 
     McDev d{};
     fill_mcdev(c, d);
-    // per-shell constants of the mixed path's shi_transfer, computed once
-    // per table set with its own arithmetic (the FP64 path evaluates them in
-    // every event)
-    if (mixed && !c->ts.sd_consts) {
-        mx::k_shi_consts<<<(nsh + 63) / 64, 64, 0, st>>>(ion_view_host(c), c->md.Ip, nsh, c->ts.sd_M, c->ts.sd_c);
-        HIPC(hipGetLastError());
+    // per-shell constants of shi_transfer and the index of its rows by draw,
+    // computed once per table set with the path's own arithmetic (the FP64
+    // path still evaluates the constants in every event, to the same bits)
+    if (!c->ts.sd_consts) {
+        HIPC(launch_shi_consts(c, ion_view_host(c), mixed, st));
         c->ts.sd_consts = true;
     }
     d.seed = seed;

@@ -35,6 +35,11 @@ struct MatDev {
 };
 
 // One table kind on the device.  Per-shell arrays are shell-major [nsh][n].
+// The ion's energy-transfer draw RN in [b/kShiBuckets, (b+1)/kShiBuckets)
+// finds its entry of a shell's inverse-path row between the entries that
+// the bucket's two ends find (k_shi_buckets).
+constexpr int kShiBuckets = 64;
+
 struct TableDev {
     int n = 0;
     int nsh = 0;
@@ -82,9 +87,10 @@ struct TableSet {
     double shi_path0 = 0.0;         // the ion's fused mean free path at its initial energy [A]
     int *sd_n = nullptr;            // [nsh] SHI_DIFF row length per shell (device)
     int64_t *sd_off = nullptr;      // [nsh+1] SHI_DIFF row offsets (device)
-    int *sd_M = nullptr;            // [2*nsh] mixed-path transfer constants (k_shi_consts)
+    int *sd_M = nullptr;            // [2*nsh] transfer constants of the path's precision (k_shi_consts)
     double *sd_c = nullptr;         // [2*nsh]
-    bool sd_consts = false;         // sd_M / sd_c filled
+    int *sd_bk = nullptr;           // [nsh][kShiBuckets + 1] transfer-row index by draw (k_shi_buckets)
+    bool sd_consts = false;         // sd_M / sd_c / sd_bk filled
     int64_t launches = 0;
     double build_seconds = 0.0;
     int64_t dcs_evals = 0;         // number of q-integrals evaluated

@@ -850,9 +850,9 @@ void free_tables(TableSet &ts)
         for (double **p : {&ts.ln_L[k], &ts.ln_hw[k], &ts.ln_lam[k]}) { if (*p) (void)hipFree(*p); *p = nullptr; }
     double **lp[] = {&ts.ln_el_imfp, &ts.ln_hole_imfp, &ts.ln_Lel, &ts.ln_LHel, &ts.ln_shi_path};
     for (double **p : lp) { if (*p) (void)hipFree(*p); *p = nullptr; }
-    void *sp[] = {ts.sd_n, ts.sd_off, ts.sd_M, ts.sd_c};
+    void *sp[] = {ts.sd_n, ts.sd_off, ts.sd_M, ts.sd_c, ts.sd_bk};
     for (void *p : sp) if (p) (void)hipFree(p);
-    ts.sd_n = nullptr; ts.sd_off = nullptr; ts.sd_M = nullptr; ts.sd_c = nullptr;
+    ts.sd_n = nullptr; ts.sd_off = nullptr; ts.sd_M = nullptr; ts.sd_c = nullptr; ts.sd_bk = nullptr;
     ts.sd_consts = false;
     ts.grids_increasing = false;
 }
@@ -1120,7 +1120,8 @@ int build_tables(const HostMat &h, const MatDev &md, TableSet &ts, hipStream_t s
         ts.shi_path0 = nfp_2d_host(gS.data(), sp.data(), (int)gS.size(), h.ion.E);
         const TableDev &td = T[TRK_TAB_SHI_DIFF];
         if (dalloc(&ts.sd_n, (size_t)nsh, err) || dalloc(&ts.sd_off, (size_t)nsh + 1, err) ||
-            dalloc(&ts.sd_M, (size_t)2 * nsh, err) || dalloc(&ts.sd_c, (size_t)2 * nsh, err))
+            dalloc(&ts.sd_M, (size_t)2 * nsh, err) || dalloc(&ts.sd_c, (size_t)2 * nsh, err) ||
+            dalloc(&ts.sd_bk, (size_t)nsh * (kShiBuckets + 1), err))
             return TRK_ERR_DEVICE;
         HIP_OK(hipMemcpy(ts.sd_n, td.dn.data(), sizeof(int) * nsh, hipMemcpyHostToDevice));
         HIP_OK(hipMemcpy(ts.sd_off, td.doff.data(), sizeof(int64_t) * (nsh + 1), hipMemcpyHostToDevice));
