@@ -83,7 +83,10 @@ typedef struct trk_material {
     double  v_f;                        /* sqrt(2 E_F / me), reference units */
     double  cut_off;                    /* [eV] */
     double  temp;                       /* [K] */
-    double  work_function;              /* [eV]; must be 0 (emission not built) */
+    double  work_function;              /* [eV]; > 0 turns on emission through the surface Z = 0
+                                           (calculate_emission, Monte_Carlo.f90:2477-2513) */
+    double  bar_length;                 /* [A] surface barrier length (needed when work_function > 0) */
+    double  bar_height;                 /* [eV] surface barrier height (> work_function) */
 
     int32_t n_dos;                      /* DOS arrays, caller-owned, length n_dos */
     const double *dos_E;                /* [eV] from the VB top, increasing */
@@ -157,6 +160,8 @@ typedef struct trk_tallies {
     double *Out_E_h;                    /* (Nt, n_atoms, max_shells) */
     double *Out_theta, *Out_theta_h;    /* (Nt+1, 180) */
     double *Out_diff_coeff;             /* (Nt) */
+    double *Out_Ne_Em, *Out_E_Em;       /* (Nt) electrons emitted through the surface, their kinetic energy */
+    double *Out_Ee_vs_E_Em;             /* (Nt, NR) their spectrum, binned on Out_R/10 */
     int64_t n_events[8];                /* out: SHI, e-inel, e-el, h-inel, h-el, Auger, radiative, photon */
     int64_t n_particles;                /* out: electrons + holes + photons created */
 } trk_tallies;
@@ -339,6 +344,13 @@ double trk_constant(int which);
  * (TRK_ERR_ARG, reason in why).  trk_ctx_create applies the same rules to its
  * copy of the descriptor. */
 int  trk_check_atomic_parameters(trk_material *m, int32_t include_photons, char *why, size_t n);
+
+/* Logistic transmission of the surface barrier, P(E) = 1/(1 + exp(gamma (E1 - E))),
+ * fitted to the Eckart barrier of the given work function [eV], length [A] and
+ * height [eV] (barrier_parameters, Monte_Carlo.f90:2094-2116).  TRK_ERR_ARG
+ * when the parameters give no finite E1 and gamma. */
+int  trk_barrier_parameters(double work_function, double bar_length, double bar_height,
+                            double *E1, double *gamma);
 
 /* f-sum and k-sum rules of a set of Drude-Lorentz oscillators
  * (sumrules, Cross_sections.f90:728-749). */

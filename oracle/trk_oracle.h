@@ -85,6 +85,8 @@ int    or_auger_pick(const or_ctx *c, double NRG, double frac, int second);
 void   or_new_angles(double phi0, double theta0, double theta, double psi, double *phi1, double *theta1);
 double or_gap_walk(const or_ctx *c, double Ehole, int down);
 double or_constant(int which);
+/* barrier_parameters, MC:2094-2116; returns 0 when E1 and gamma are finite */
+int    or_barrier_parameters(double W, double bar_length, double H, double *E1, double *gamma);
 
 /* Monte Carlo (do_Monte_Carlo in the reference's global time order).  If
  * trace != NULL each processed event appends 12 doubles, the record layout of

@@ -68,6 +68,9 @@ struct or_ctx {
     int tables_ready;
     /* fused totals (How_many_electrons) */
     double *El_IMFP, *El_EMFP, *Hole_IMFP, *Hole_EMFP, *Phot_IMFP, *SHI_path;
+    /* surface barrier (emission when em_W > 0): work function, height and
+     * the logistic transmission's E1 and gamma */
+    double em_W, em_H, em_E1, em_gamma;
     /* time grid */
     int32_t Nt;
     double *tim_glob;

@@ -68,6 +68,7 @@ typedef struct {
     trk_tallies *out;
     double *trace; int64_t tcap, tn;
     int64_t n_events[8];
+    double *em; int nem, capem;       /* kinetic energies of the emitted electrons (Em_electrons) */
 } or_mc;
 
 /* ---------------------------------------------------------------- heap */
@@ -726,6 +727,49 @@ static void shi_event(or_mc *S)
 }
 
 /* ---------------------------------------------------------------- electron */
+/* barrier_parameters, MC:2094-2116 (g_h is the reduced Planck constant) */
+int or_barrier_parameters(double W, double bar_length, double H, double *E1, double *gamma)
+{
+    double Em_L = bar_length * 1.0e-10;
+    double Em_B = 2.0 * H - W + 2.0 * sqrt(H * H - H * W);
+    double Em_ksi = 0.5 * sqrt(8.0 * OR_ME * Em_L * Em_L * Em_B * OR_E / ((2.0 * OR_PI * OR_H) * (2.0 * OR_PI * OR_H)) - 1.0);
+    double Em_bb = cosh(2.0 * OR_PI * Em_ksi);
+    double Em_delta = 2.0 * OR_PI * Em_L * sqrt(2.0 * OR_ME * OR_E) / (2.0 * OR_PI * OR_H);
+    double e1 = H + 2.0 * sqrt(H * (H - W)) * (acosh(Em_bb) / (Em_delta * (sqrt(H) + sqrt(H - W))) - 1.0);
+    double g1 = Em_delta * (sqrt(e1) + sqrt(e1 - W));
+    double g2 = Em_delta * (sqrt(e1) - sqrt(e1 - W));
+    *E1 = e1;
+    *gamma = (g1 * sinh(g1) + 2.0 * g2 * sinh(g2)) / (sqrt(e1 * (e1 - W)) * (Em_bb + cosh(g1)));
+    return (isfinite(*E1) && isfinite(*gamma) && W > 0.0 && bar_length > 0.0 && H > W) ? 0 : 1;
+}
+
+/* calculate_emission, MC:2477-2513, for an electron whose collision point
+ * has Z < 0: emitted (no further event, its kinetic energy outside E - W
+ * recorded) or, if it moves outward, turned back */
+static void emission(or_mc *S, p_el *e, uint32_t *nd)
+{
+    const or_ctx *c = S->c;
+    int out = 0;
+    double Ekin = e->E - c->em_W;
+    if (e->E >= 1.5 * c->em_H) {
+        out = 1;
+    } else {
+        double RN = rn(S, e->id, nd);
+        double Em_Penetr = 1.0 / (1.0 + EXP(c->em_gamma * (c->em_E1 - e->E)));
+        if ((Ekin > 0.0) && (RN < Em_Penetr)) out = 1;
+        else if (COS(e->theta) < 0.0) e->theta = OR_PI - e->theta;
+    }
+    if (out) {
+        e->tn = 1.0e30;
+        e->L = 1.0e30;
+        if (S->nem == S->capem) {
+            S->capem = S->capem ? 2 * S->capem : 256;
+            S->em = (double *)realloc(S->em, sizeof(double) * (size_t)S->capem);
+        }
+        S->em[S->nem++] = Ekin;
+    }
+}
+
 /* Electron_Monte_Carlo, MC:2253-2474 */
 static void electron_event(or_mc *S, int idx)
 {
@@ -802,6 +846,7 @@ static void electron_event(or_mc *S, int idx)
     e->E = Eel - dE; e->t0 = e->tn; e->X = X; e->Y = Y; e->Z = Z; e->L = MFP; e->theta = theta1; e->phi = phi1;
     e->tn = t_next(e->t0, MFP, vel_el(e->E));
     cut_el(c, e);
+    if (c->em_W > 0.0 && Z < 0.0) emission(S, e, &nd);
     e->nd = nd;
     trace_ev(S, kind, id, ev, t_ev, e->E, 0.0, X, Y, Z, e->tn, nd);
     hpush(S, e->tn, K_EL, idx);
@@ -1069,7 +1114,7 @@ static void photon_event(or_mc *S, int idx)
 /* ---------------------------------------------------------------- statistics */
 #define IX2(i, j, N1) ((int64_t)(i) + (int64_t)(N1) * (int64_t)(j))
 
-/* Calculated_statistics, MC:881-1110 (field and emission parts are zero) */
+/* Calculated_statistics, MC:881-1110 (the field parts are zero) */
 static void statistics(or_mc *S, int i, double tim)
 {
     or_ctx *c = S->c;
@@ -1091,6 +1136,23 @@ static void statistics(or_mc *S, int i, double tim)
     }
     if (o->Out_E_e) o->Out_E_e[i0] += Se;
     if (o->Out_E_at) o->Out_E_at[i0] += S->At_NRG;
+    /* emitted electrons, MC:958-959, 1101-1109, on the energy grid Out_R/10 */
+    if (c->em_W > 0.0) {
+        double Sem = 0.0;
+        for (int k = 0; k < S->nem; ++k) Sem = Sem + S->em[k];
+        if (o->Out_Ne_Em) o->Out_Ne_Em[i0] += (double)S->nem;
+        if (o->Out_E_Em) o->Out_E_Em[i0] += Sem;
+        if (o->Out_Ee_vs_E_Em && S->nem > 0) {
+            double *Out_E = (double *)malloc(sizeof(double) * (size_t)NR);
+            for (int j = 0; j < NR; ++j) Out_E[j] = S->R[j] / 10.0;
+            for (int k = 0; k < S->nem; ++k) {
+                int j = or_find_mono_1d(Out_E, NR, S->em[k]);
+                if (j > 1) o->Out_Ee_vs_E_Em[IX2(i0, j - 1, Nt)] += (1.0 / (Out_E[j - 1] - Out_E[j - 2])) / (double)S->nem;
+                else o->Out_Ee_vs_E_Em[IX2(i0, j - 1, Nt)] += (1.0 / Out_E[j - 1]) / (double)S->nem;
+            }
+            free(Out_E);
+        }
+    }
     if (o->Out_E_h) {
         int ms = o->max_shells;
         for (int a = 0; a < c->m.n_atoms; ++a)
@@ -1253,6 +1315,7 @@ static void one_iteration(or_mc *S)
     S->nph = 0;
     S->nph_alive = 0;
     S->nh = 0;
+    S->nem = 0;
     S->At_NRG = 0.0;
     p_shi *I = &S->shi;
     memset(I, 0, sizeof(*I));
@@ -1311,7 +1374,7 @@ int or_run_mc(or_ctx *c, int64_t nmc, uint64_t seed, int64_t first_iter,
     for (int k = 0; k < 8; ++k) out->n_events[k] += S.n_events[k];
     out->n_particles += nparticles / 2 + 0;
     if (trace_n) *trace_n = S.tn;
-    free(S.el); free(S.ho); free(S.ph); free(S.heap);
+    free(S.el); free(S.ho); free(S.ph); free(S.heap); free(S.em);
     if (c->err_numeric) { snprintf(c->err, sizeof c->err, "NaN reached a table search"); return TRK_ERR_NUMERIC; }
     return 0;
 }
@@ -1336,12 +1399,13 @@ int or_run_mc_threads(or_ctx *c, int64_t nmc, uint64_t seed, int64_t first_iter,
         double **dst[] = {&L.Out_ne, &L.Out_Ee, &L.Out_nphot, &L.Out_Ephot, &L.Out_Ee_vs_E, &L.Out_Elat,
                           &L.Out_nh, &L.Out_Eh, &L.Out_Ehkin, &L.Out_tot_Ne, &L.Out_tot_Nphot, &L.Out_tot_E,
                           &L.Out_E_e, &L.Out_E_phot, &L.Out_E_at, &L.Out_E_h, &L.Out_theta, &L.Out_theta_h,
-                          &L.Out_Eh_vs_E};
+                          &L.Out_Eh_vs_E, &L.Out_Ne_Em, &L.Out_E_Em, &L.Out_Ee_vs_E_Em};
         double *const src[] = {out->Out_ne, out->Out_Ee, out->Out_nphot, out->Out_Ephot, out->Out_Ee_vs_E,
                                out->Out_Elat, out->Out_nh, out->Out_Eh, out->Out_Ehkin, out->Out_tot_Ne,
                                out->Out_tot_Nphot, out->Out_tot_E, out->Out_E_e, out->Out_E_phot, out->Out_E_at,
-                               out->Out_E_h, out->Out_theta, out->Out_theta_h, out->Out_Eh_vs_E};
-        const size_t len[] = {n2, n2, n2, n2, n2, n2, nh, nh, nh, n1, n1, n1, n1, n1, n1, ns, nth, nth, ndos};
+                               out->Out_E_h, out->Out_theta, out->Out_theta_h, out->Out_Eh_vs_E,
+                               out->Out_Ne_Em, out->Out_E_Em, out->Out_Ee_vs_E_Em};
+        const size_t len[] = {n2, n2, n2, n2, n2, n2, nh, nh, nh, n1, n1, n1, n1, n1, n1, ns, nth, nth, ndos, n1, n1, n2};
         const int na_ = (int)(sizeof len / sizeof len[0]);
         for (int k = 0; k < na_; ++k) *dst[k] = src[k] ? (double *)calloc(len[k], sizeof(double)) : NULL;
         L.Out_diff_coeff = NULL;
@@ -1362,7 +1426,7 @@ int or_run_mc_threads(or_ctx *c, int64_t nmc, uint64_t seed, int64_t first_iter,
             for (int k = 0; k < 8; ++k) ev[k] += S.n_events[k];
         }
         for (int k = 0; k < na_; ++k) free(*dst[k]);
-        free(S.el); free(S.ho); free(S.ph); free(S.heap);
+        free(S.el); free(S.ho); free(S.ph); free(S.heap); free(S.em);
     }
     for (int k = 0; k < 8; ++k) out->n_events[k] += ev[k];
     if (c->err_numeric) { snprintf(c->err, sizeof c->err, "NaN reached a table search"); return TRK_ERR_NUMERIC; }

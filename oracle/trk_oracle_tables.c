@@ -1159,6 +1159,11 @@ or_ctx *or_create(const trk_material *mat, const trk_ion *ion, const trk_options
     c->Zt = sz / sp;
     c->e_emax = (opt->e_grid_Emax > 0.0) ? opt->e_grid_Emax : ((175.6e6 * 2.0) / 1836.0);
     c->shi_Zeff0 = or_shi_zeff(c, ion->E);
+    if (mat->work_function > 0.0 &&
+        or_barrier_parameters(mat->work_function, mat->bar_length, mat->bar_height, &c->em_E1, &c->em_gamma) == 0) {
+        c->em_W = mat->work_function;
+        c->em_H = mat->bar_height;
+    }
     or_build_time_grid(c);
     return c;
 }

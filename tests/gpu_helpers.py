@@ -63,6 +63,8 @@ TALLY_TOL = {  # relative tolerance per tally (fixed-point 2^-40 eV sums on the 
     "Out_tot_E": 1e-10, "Out_E_e": 1e-10, "Out_E_phot": 1e-10, "Out_E_at": 1e-10, "Out_E_h": 1e-10,
     # normalised per iteration: count * (w / n) on the GPU, n additions of w / n in the oracle
     "Out_Ee_vs_E": 1e-11, "Out_Eh_vs_E": 1e-11, "Out_theta": 1e-11, "Out_theta_h": 1e-11, "Out_diff_coeff": 1e-11,
+    # emitted electrons: count, kinetic energy sum, spectrum normalised per iteration
+    "Out_Ne_Em": 1e-12, "Out_E_Em": 1e-10, "Out_Ee_vs_E_Em": 1e-11,
 }
 
 

@@ -22,7 +22,7 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 DRIVER = os.path.join(ROOT, "trekis-3_amd", "fortran", "_build", "bridge_driver")
 ORDER = ["Out_ne", "Out_Ee", "Out_nphot", "Out_Ephot", "Out_Ee_vs_E", "Out_Eh_vs_E", "Out_Elat", "Out_nh", "Out_Eh",
          "Out_Ehkin", "Out_tot_Ne", "Out_tot_Nphot", "Out_tot_E", "Out_E_e", "Out_E_phot", "Out_E_at", "Out_E_h",
-         "Out_theta", "Out_theta_h", "Out_diff_coeff"]
+         "Out_theta", "Out_theta_h", "Out_diff_coeff", "Out_Ne_Em", "Out_E_Em", "Out_Ee_vs_E_Em"]
 
 
 def _descriptor(m, ion, opt, nmc, seed, Nt, R, V, ms):
@@ -44,7 +44,7 @@ def _descriptor(m, ion, opt, nmc, seed, Nt, R, V, ms):
     for k in range(m.n_phonon):
         f8(m.ph_E0[k], m.ph_A[k], m.ph_Gamma[k])
     f8(m.Dens, m.At_Dens, m.Vsound, m.Layer, m.N_VB_el, m.El_eff_mass, m.hole_mass, m.E_F, m.v_f, m.cut_off, m.temp,
-       m.work_function)
+       m.work_function, m.bar_length, m.bar_height)
     i4(m.n_dos)
     for f in ("dos_E", "dos_DOS", "dos_int_DOS", "dos_k", "dos_Eff_m", "dos_k_inv", "dos_Eff_m_inv"):
         f8(*[getattr(m, f)[k] for k in range(m.n_dos)])
@@ -58,7 +58,7 @@ def _descriptor(m, ion, opt, nmc, seed, Nt, R, V, ms):
 
 @pytest.mark.skipif(not os.path.exists(DRIVER), reason="the Fortran bridge driver is built where the reference's "
                                                        "Objects.f90 is (make -C trekis-3_amd/fortran)")
-@pytest.mark.parametrize("name", ["test", "test_metal"])
+@pytest.mark.parametrize("name", ["test", "test_metal", "opt_emission"])
 def test_fortran_do_monte_carlo_equals_c_abi(name, tmp_path):
     lib = tg.load_library()
     mat, ion, opt, run = tg.config_material(name)

@@ -110,3 +110,21 @@ def test_mixed_repeatable():
     assert np.array_equal(a.n_events, b.n_events)
     for k in list(RADIAL) + ["Out_tot_E", "Out_theta", "Out_Ee_vs_E"]:
         assert np.array_equal(a[k], b[k]), k
+
+
+def test_mixed_emission_within_1pct():
+    """Emission through the surface (opt_emission) on the mixed path: the
+    numbers and kinetic energies of the emitted electrons at the last three
+    quarter nodes within 1 % of the FP64 path's on the same 256 iterations,
+    where each holds thousands of emissions."""
+    ctx, orc, mat, ion, opt, run, R, V = setup("opt_emission")
+    nmc = 256
+    ref = ctx.run_mc(nmc, run["seed"], R, V)
+    mix = _mixed_context("opt_emission").run_mc(nmc, run["seed"], R, V)
+    Nt = ref["Out_Ne_Em"].shape[0]
+    for k in ("Out_Ne_Em", "Out_E_Em"):
+        for i in (Nt // 2, (3 * Nt) // 4, Nt - 1):
+            a, b = float(ref[k][i]), float(mix[k][i])
+            say(f"[mixed] opt_emission {k}[{i}]: {a:.9e} vs {b:.9e}")
+            assert a > 0 and abs(b - a) <= 1e-2 * a, (k, i)
+    assert ref["Out_Ne_Em"][-1] >= 1000

@@ -8,8 +8,13 @@ Save_data_for_distributions), particle-number conservation (every
 ionisation, Auger decay and photoabsorption makes one electron and one hole,
 so Sum_j ne_j/V_j == Sum_j nh_j/V_j == Out_tot_Ne), the reference's event
 order (non-decreasing event times within an iteration), per-particle event
-numbering, and bitwise determinism of a seeded run."""
+numbering, and bitwise determinism of a seeded run.  With emission through
+the surface (opt_emission), the rules of calculate_emission (MC:2477-2513)
+are read back from the event trace: which collisions outside the surface
+emit, the emitted fraction against the barrier's transmission, and the
+emitted-electron tallies formed from the emissions the trace records."""
 import ctypes
+import math
 
 import numpy as np
 import pytest
@@ -17,7 +22,7 @@ import pytest
 import trekis_gpu as tg
 from oracle import Oracle, dp
 
-CONFIGS = ["test", "test_metal"]
+CONFIGS = ["test", "test_metal", "opt_emission"]
 
 
 @pytest.fixture(scope="module", params=CONFIGS)
@@ -120,3 +125,45 @@ def test_threaded_run_matches_sequential(runs):
     assert (a.n_events == b.n_events).all()
     for k in ("Out_ne", "Out_Ee", "Out_nh", "Out_Elat", "Out_tot_E", "Out_theta"):
         np.testing.assert_allclose(b[k], a[k], rtol=1e-11, atol=1e-11 * abs(a[k]).max())
+
+
+def test_emission_rules(runs):
+    """Electron collisions at Z < 0 emit always at E >= 1.5 H, never at
+    E <= W, and in between with the logistic transmission (the emitted
+    count within 4 sigma of the summed probabilities); collisions inside
+    never emit; an emitted electron has no later event; Out_Ne_Em and
+    Out_E_Em at each node count and sum E - W over the emissions before it,
+    and every iteration with an emission adds exactly one to the integral
+    of Out_Ee_vs_E_Em over the energy grid Out_R/10."""
+    if runs["name"] != "opt_emission":
+        pytest.skip("emission is on only in opt_emission")
+    a, mat, R, tim = runs["a"], runs["mat"], runs["R"], runs["tim"]
+    W, H = mat.work_function, mat.bar_height
+    E1, g = ctypes.c_double(), ctypes.c_double()
+    assert runs["o"].lib.or_barrier_parameters(W, mat.bar_length, H, ctypes.byref(E1), ctypes.byref(g)) == 0
+    tr = a.trace
+    el = tr[(tr[:, 3] == 1) | (tr[:, 3] == 2)]
+    em = el[:, 10] == 1e30
+    out = el[:, 9] < 0.0
+    assert not em[~out].any()
+    E = el[:, 5]
+    assert em[out & (E >= 1.5 * H)].all() and (out & (E >= 1.5 * H)).sum() > 10
+    assert not em[out & (E <= W)].any()
+    mid = out & (E > W) & (E < 1.5 * H)
+    P = 1.0 / (1.0 + np.exp(g.value * (E1.value - E[mid])))
+    assert mid.sum() > 50
+    assert abs(em[mid].sum() - P.sum()) < 4.0 * math.sqrt((P * (1 - P)).sum()) + 1.0
+    # no event of an emitted electron after its emission
+    ids = el[:, 1].view(np.uint64)
+    for row in el[em]:
+        same = (el[:, 0] == row[0]) & (ids == row[1:2].view(np.uint64)[0])
+        assert el[same, 2].max() == row[2]
+    RE = R / 10.0
+    width = np.concatenate([[RE[0]], np.diff(RE)])
+    for i, t in enumerate(tim):
+        before = em & (el[:, 4] < t)
+        assert a["Out_Ne_Em"][i] == before.sum()
+        np.testing.assert_allclose(a["Out_E_Em"][i], (E[before] - W).sum(), rtol=1e-12, atol=0)
+        n_iter = len(np.unique(el[before, 0]))
+        np.testing.assert_allclose((a["Out_Ee_vs_E_Em"][i] * width).sum(), n_iter, rtol=1e-12)
+    assert a["Out_Ne_Em"][-1] > 0

@@ -30,6 +30,11 @@ the reference's expressions:
 * the valence-hole gap walk (check_hole_parameters, MC:696-719): brute
   force over a small DOS with empty levels -- the nearest nonempty level
   below (hole-electron) or above (hole-lattice) the bracket of the energy.
+* barrier_parameters (MC:2094-2116): the logistic transmission
+  1/(1 + exp(gamma (E1 - E))) of the surface is fitted to Eckart's
+  transmission through a barrier of height H, work function W and length L
+  (Phys. Rev. 35, 1303 (1930)): E1 is where it passes one half, gamma four
+  times its slope there.
 """
 import ctypes
 import math
@@ -296,3 +301,42 @@ def test_brandt_kitagawa_cross_section_brackets(lib):
                     assert 0.0 < Zeff < Zp
                     assert Zeff * Zeff * p < b < Zp * Zp * p, (s, hw, Zeff)
         o.close()
+
+
+def test_surface_barrier_is_the_eckart_midpoint(lib):
+    """E1 and gamma of barrier_parameters against Eckart's transmission
+    T(E) = (cosh(a + b) - cosh(a - b)) / (cosh(a + b) + cosh(2 pi d)) with
+    a = L sqrt(2 m E) / hbar, b = L sqrt(2 m (E - W)) / hbar and
+    d = sqrt(2 m (sqrt(H) + sqrt(H - W))^2 L^2 / h^2 - 1/4), written here
+    from the textbook form with CODATA constants: T(E1) = 1/2 to 0.3 % and
+    gamma = 4 T'(E1) to 1 %.  The library's host copy gives the oracle's
+    numbers bit for bit and both refuse a barrier lower than the work
+    function or of no length."""
+    import oracle as orc
+    olib = orc.load()
+    me = 9.1093837015e-31
+    h = 2.0 * math.pi * HBAR
+
+    def eckart(E, W, L, H):
+        Lm = L * 1e-10
+        a = Lm * math.sqrt(2.0 * me * E * EV) / HBAR
+        b = Lm * math.sqrt(2.0 * me * (E - W) * EV) / HBAR
+        B = (math.sqrt(H) + math.sqrt(H - W)) ** 2
+        d = math.sqrt(2.0 * me * B * EV * Lm * Lm / (h * h) - 0.25)
+        return (math.cosh(a + b) - math.cosh(a - b)) / (math.cosh(a + b) + math.cosh(2.0 * math.pi * d))
+
+    for W, L, H in ((4.5, 10.0, 6.18), (4.5, 10.0, 10.0), (5.0, 10.0, 6.18), (4.0, 5.0, 8.0), (3.0, 20.0, 5.0)):
+        E1, g = ctypes.c_double(), ctypes.c_double()
+        assert olib.or_barrier_parameters(W, L, H, ctypes.byref(E1), ctypes.byref(g)) == 0
+        E1l, gl = ctypes.c_double(), ctypes.c_double()
+        assert lib.trk_barrier_parameters(W, L, H, ctypes.byref(E1l), ctypes.byref(gl)) == 0
+        assert (E1l.value, gl.value) == (E1.value, g.value)
+        assert W < E1.value < H * 1.5
+        assert abs(eckart(E1.value, W, L, H) - 0.5) < 3e-3, (W, L, H)
+        dE = 1e-5
+        slope = (eckart(E1.value + dE, W, L, H) - eckart(E1.value - dE, W, L, H)) / (2 * dE)
+        assert g.value == pytest.approx(4.0 * slope, rel=1e-2), (W, L, H)
+    for W, L, H in ((4.5, 10.0, 4.0), (4.5, 0.0, 6.0), (0.0, 10.0, 6.0)):
+        E1, g = ctypes.c_double(), ctypes.c_double()
+        assert olib.or_barrier_parameters(W, L, H, ctypes.byref(E1), ctypes.byref(g)) != 0
+        assert lib.trk_barrier_parameters(W, L, H, ctypes.byref(E1), ctypes.byref(g)) != 0

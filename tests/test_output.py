@@ -130,6 +130,37 @@ def test_output_files_follow_save_output(tmp_path):
     assert not (tmp_path / "heavy" / "Radial_photon_density[1_cm^-3].txt").exists()
 
 
+def test_emission_files_follow_save_output(tmp_path):
+    """With emission on (work function > 0): the emitted electrons' spectrum
+    file (SO:592-615) has the time header of the radial files, rows
+    labelled by Out_R/10 in f9.1 and the normalised spectrum; the second and
+    fourth columns of Total_numbers.txt (SO:630-640) are the normalised
+    emitted number and energy.  Without emission the file is not written."""
+    from trekis_gpu import output
+    t = _fake_tallies(Nt=5, NR=4)
+    r = np.random.default_rng(11)
+    t["Out_Ne_Em"] = np.cumsum(r.integers(0, 5, 5)).astype(float)
+    t["Out_E_Em"] = np.cumsum(r.uniform(0.0, 50.0, 5))
+    t["Out_Ee_vs_E_Em"] = r.uniform(0.0, 1.0, (5, 4))
+    E, D = configs.dos_insulator()
+    mat = inputs.read_cdf(TWO_ELEMENTS, layer=100.0, dos=(E, D))
+    mat.work_function, mat.bar_length, mat.bar_height = 4.5, 10.0, 6.18
+    times = [10.0, 20.0, 30.0, 40.0, 50.0]
+    R = [1.0, 3.0, 10.0, 100.0]
+    names = output.write_output(str(tmp_path), t, 4, mat, times, R, include_photons=False)
+    assert "Emitted_electron_distribution_vs_E[1_eV].txt" in names
+    head, lab, val = _read_matrix(tmp_path / "Emitted_electron_distribution_vs_E[1_eV].txt")
+    assert head == "#Energy[eV] " + "".join(f"{x:10.2f}[fs]   " for x in times) + " "
+    assert lab.tolist() == [0.1, 0.3, 1.0, 10.0]
+    assert np.allclose(val, (t["Out_Ee_vs_E_Em"] / 4.0).T, rtol=1e-15, atol=0)
+    rows = np.loadtxt(tmp_path / "Total_numbers.txt", comments="#")
+    assert rows.shape == (5, 5)
+    assert np.allclose(rows[:, 2], t["Out_Ne_Em"] / 4.0, rtol=1e-15) and np.allclose(rows[:, 4], t["Out_E_Em"] / 4.0)
+    mat.work_function = 0.0
+    names = output.write_output(str(tmp_path / "none"), t, 4, mat, times, R, include_photons=False)
+    assert "Emitted_electron_distribution_vs_E[1_eV].txt" not in names
+
+
 def test_photon_paths_from_cross_sections():
     """EPDL branch of Tot_Phot_IMFP (CS:855-877): a core shell's photon path
     is 1/(n x_a sigma), the valence band's 1/(n sigma_VB); no absorption

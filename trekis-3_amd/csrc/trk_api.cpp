@@ -54,7 +54,17 @@ static int validate(const trk_material *m, const trk_ion *ion, const trk_options
     if (o->kind_of_EMFP < -1 || o->kind_of_EMFP > 1) { why = "kind_of_EMFP must be -1, 0 or 1 (DSF is not built)"; return 1; }
     if (m->n_dos < 2 || !m->dos_E || !m->dos_DOS || !m->dos_int_DOS || !m->dos_Eff_m) { why = "a VB density of states is required"; return 1; }
     if (m->El_eff_mass == 0.0 && (!m->dos_k || !m->dos_k_inv || !m->dos_Eff_m_inv)) { why = "El_eff_mass = 0 needs k and Eff_m arrays"; return 1; }
-    if (m->work_function > 0.0) { why = "surface emission (work_function > 0) is not built"; return 1; }
+    if (m->work_function > 0.0) {
+        // the reader turns emission off when a barrier dimension is not
+        // positive (RF:378-381); a caller passing one is refused instead
+        double E1 = 0.0, g = 0.0;
+        if (!(m->bar_length > 0.0) || !(m->bar_height > m->work_function) ||
+            trk_barrier_parameters(m->work_function, m->bar_length, m->bar_height, &E1, &g) != 0) {
+            why = "emission (work_function > 0) needs bar_length > 0 and bar_height > work_function, "
+                  "with a finite barrier transmission (barrier_parameters, MC:2094-2116)";
+            return 1;
+        }
+    }
     if (!(m->At_Dens > 0.0) || !(m->Layer > 0.0)) { why = "At_Dens and Layer must be positive"; return 1; }
     if (ion->Zat < 1 || !(ion->Mass > 0.0) || !(ion->E > 0.0)) { why = "invalid ion"; return 1; }
     if (ion->Kind_ion != 0 && ion->Kind_ion != 1) { why = "Kind_ion must be 0 (point charge) or 1 (Brandt-Kitagawa)"; return 1; }

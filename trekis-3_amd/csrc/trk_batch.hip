@@ -453,9 +453,10 @@ static int check_outs(trk_batch *b, trk_tallies *const *outs)
         if (o->Nt != c->hm.Nt || o->NR != b->NR)
             return berr(b, TRK_ERR_ARG, "tallies of impact %d have Nt=%d NR=%d, the impact has %d and %d", i, o->Nt,
                         o->NR, c->hm.Nt, b->NR);
-        if (o->Out_Ee_vs_E || o->Out_Eh_vs_E || o->Out_theta || o->Out_theta_h || o->Out_diff_coeff)
-            return berr(b, TRK_ERR_ARG, "the per-iteration normalised tallies (spectra, angles, diffusion) are "
-                                        "made by trk_run_mc, not by batches: pass NULL for them");
+        if (o->Out_Ee_vs_E || o->Out_Eh_vs_E || o->Out_theta || o->Out_theta_h || o->Out_diff_coeff ||
+            o->Out_Ne_Em || o->Out_E_Em || o->Out_Ee_vs_E_Em)
+            return berr(b, TRK_ERR_ARG, "the per-iteration normalised tallies (spectra, angles, diffusion, "
+                                        "emitted electrons) are made by trk_run_mc, not by batches: pass NULL for them");
         int ms = 0;
         for (int a = 0; a < c->hm.m.n_atoms; ++a) ms = c->hm.m.n_shells[a] > ms ? c->hm.m.n_shells[a] : ms;
         if ((o->Out_nh || o->Out_Eh || o->Out_Ehkin || o->Out_E_h) && o->max_shells < ms)

@@ -290,6 +290,30 @@ double trk_w_plasma(double density_m3, double mass_kg)
     return ((density_m3 * trk::kE) * trk::kE) / (trk::kE0 * trk::kMe);
 }
 
+// barrier_parameters, Monte_Carlo.f90:2094-2116: the energy E1 at which the
+// Eckart barrier of height H over a length L passes half of the electrons,
+// and the slope gamma of the logistic 1/(1 + exp(gamma (E1 - E))) there
+int trk_barrier_parameters(double W, double bar_length, double H, double *E1_out, double *gamma_out)
+{
+    using namespace trk;
+    if (!E1_out || !gamma_out) return TRK_ERR_ARG;
+    const double L = bar_length * 1.0e-10;
+    const double twopi_h = (2.0 * kPi) * kH;
+    const double B = ((2.0 * H) - W) + 2.0 * std::sqrt(H * H - H * W);
+    const double ksi = 0.5 * std::sqrt((((((8.0 * kMe) * L) * L) * B) * kE) / (twopi_h * twopi_h) - 1.0);
+    const double bb = std::cosh((2.0 * kPi) * ksi);
+    const double delta = (((2.0 * kPi) * L) * std::sqrt((2.0 * kMe) * kE)) / twopi_h;
+    const double E1 = H + (2.0 * std::sqrt(H * (H - W))) *
+                              (std::acosh(bb) / (delta * (std::sqrt(H) + std::sqrt(H - W))) - 1.0);
+    const double g1 = delta * (std::sqrt(E1) + std::sqrt(E1 - W));
+    const double g2 = delta * (std::sqrt(E1) - std::sqrt(E1 - W));
+    const double gamma = (g1 * std::sinh(g1) + (2.0 * g2) * std::sinh(g2)) /
+                         (std::sqrt(E1 * (E1 - W)) * (bb + std::cosh(g1)));
+    *E1_out = E1;
+    *gamma_out = gamma;
+    return (std::isfinite(E1) && std::isfinite(gamma) && W > 0.0 && bar_length > 0.0 && H > W) ? 0 : TRK_ERR_ARG;
+}
+
 // sumrules, Cross_sections.f90:728-749
 int trk_sumrules(const double *A, const double *E0, const double *Gamma, int32_t n,
                  double x_min, double Omega, double *ksum, double *fsum)

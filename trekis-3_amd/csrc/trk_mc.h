@@ -127,6 +127,16 @@ struct McDev {
     uint32_t iter0;
     int round_events;
     GroupCtl *gctl;
+    // emission through the surface Z = 0 (em_W, the work function, > 0): the
+    // barrier height, the E1 and gamma of its logistic transmission, and the
+    // per-iteration tallies of the emitted electrons while S.on (indexed like
+    // S): counts [B][Nt], kinetic energies [B][Nt][2] (fixed point) and
+    // spectrum [B][Nt][NR] on the energy grid RE = Out_R/10, all difference
+    // arrays over time
+    double em_W, em_H, em_E1, em_gamma;
+    const double *RE;
+    int *em_n, *em_h;
+    unsigned long long *em_e;
 };
 
 enum { EV_SHI = 0, EV_E_INEL, EV_E_EL, EV_H_INEL, EV_H_EL, EV_AUGER, EV_RAD, EV_PHOT };

@@ -65,6 +65,47 @@ __global__ void k_prefix_fx(unsigned long long *a, int B, int Nt)
     }
 }
 
+// The energy grid of the emitted electrons' spectrum, Out_E = Out_R/10 (MC:940-941).
+__global__ void k_tenth(double *RE, const double *R, int NR)
+{
+    int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j < NR) RE[j] = R[j] / 10.0;
+}
+
+// Prefix over time of the emitted electrons' kinetic energies [B][Nt][2]:
+// the split words of each node are combined and the running sum is left as
+// one 128-bit integer (low, high) in units of 2^-40 eV.
+__global__ void k_prefix_fx128(unsigned long long *a, int B, int Nt)
+{
+    int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= B) return;
+    unsigned long long *p = a + (size_t)b * Nt * 2;
+    __int128 run = 0;
+    for (int i = 0; i < Nt; ++i) {
+        run += fx_read(p + 2 * i);
+        i128_store(p + 2 * i, run);
+    }
+}
+
+// Out_Ne_Em and Out_E_Em (MC:958-959): per node, the iterations' emitted
+// counts (after their prefix) and kinetic energies (after k_prefix_fx128)
+// summed exactly, then added to the accumulators.
+__global__ void k_emit_sums(double *acc_n, double *acc_e, const int *n, const unsigned long long *e, int B, int Nt)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= Nt) return;
+    long long cn = 0;
+    __int128 ce = 0;
+    for (int b = 0; b < B; ++b) {
+        cn += n[(size_t)b * Nt + i];
+        ce += i128_load(e + 2 * ((size_t)b * Nt + i));
+    }
+    const long long hi = (long long)(ce >> 40);
+    const __int128 lo = ce - ((__int128)hi << 40);
+    acc_n[i] += (double)cn;
+    acc_e[i] += (double)hi + (double)(long long)lo * 0x1.0p-40;
+}
+
 // acc(i, j) += sum over iterations of count * (w_j / norm), in iteration order.
 // wmode 0: w = 1 (angle histograms); 1: w from the radial grid used as an
 // energy grid (MC:1024-1029); 2: w from the DOS grid (MC:1033-1040).
@@ -242,6 +283,13 @@ struct McState {
     Spec S{};
     int spec_B = 0, spec_Nt = 0, spec_NR = 0, spec_ND = 0;
     double *acc_eve = nullptr, *acc_ehve = nullptr, *acc_th = nullptr, *acc_thh = nullptr, *acc_dc = nullptr;
+    // emitted electrons per iteration ([B][Nt] counts, [B][Nt][2] energies,
+    // [B][Nt][NR] spectrum), their accumulators and the spectrum's grid
+    int *em_n = nullptr, *em_h = nullptr;
+    unsigned long long *em_e = nullptr;
+    double *acc_emn = nullptr, *acc_eme = nullptr, *acc_emh = nullptr;
+    double *d_RE = nullptr;
+    int RE_cap = 0;
 };
 
 #define HIPC(x)                                                                          \
@@ -280,10 +328,14 @@ static void free_list(PList &l)
 static void free_spec(McState *s)
 {
     void *ps[] = {s->S.nel, s->S.nvb, s->S.dcn, s->S.eve, s->S.ehve, s->S.th, s->S.thh, s->S.dcs,
-                  s->acc_eve, s->acc_ehve, s->acc_th, s->acc_thh, s->acc_dc};
+                  s->acc_eve, s->acc_ehve, s->acc_th, s->acc_thh, s->acc_dc,
+                  s->em_n, s->em_h, s->em_e, s->acc_emn, s->acc_eme, s->acc_emh};
     for (void *p : ps) if (p) (void)hipFree(p);
     s->S = Spec{};
     s->acc_eve = s->acc_ehve = s->acc_th = s->acc_thh = s->acc_dc = nullptr;
+    s->em_n = s->em_h = nullptr;
+    s->em_e = nullptr;
+    s->acc_emn = s->acc_eme = s->acc_emh = nullptr;
     s->spec_B = 0;
 }
 
@@ -293,7 +345,7 @@ void mc_free(McState *s)
     free_spec(s);
     for (auto &a : s->L) for (auto &l : a) free_list(l);
     void *ps[] = {s->d_counts, s->d_overflow, s->d_counters, s->T.cnt_e, s->T.E_e, s->T.cnt_h, s->T.E_h,
-                  s->T.Ek_h, s->T.cnt_p, s->T.E_p, s->T.Elat, s->d_tim, s->d_R, s->d_trace, s->d_trace_n};
+                  s->T.Ek_h, s->T.cnt_p, s->T.E_p, s->T.Elat, s->d_tim, s->d_R, s->d_RE, s->d_trace, s->d_trace_n};
     for (void *p : ps) if (p) (void)hipFree(p);
     if (s->ev0) (void)hipEventDestroy(s->ev0);
     if (s->ev1) (void)hipEventDestroy(s->ev1);
@@ -341,8 +393,22 @@ static int ensure_spec(trk_ctx *c, McState *s, int B, int Nt, int NR, int ND)
     HIPC(hipMalloc((void **)&s->acc_th, (size_t)(Nt + 1) * 180 * sizeof(double)));
     HIPC(hipMalloc((void **)&s->acc_thh, (size_t)(Nt + 1) * 180 * sizeof(double)));
     HIPC(hipMalloc((void **)&s->acc_dc, (size_t)Nt * sizeof(double)));
+    HIPC(hipMalloc((void **)&s->em_n, bt * sizeof(int)));
+    HIPC(hipMalloc((void **)&s->em_h, bt * NR * sizeof(int)));
+    HIPC(hipMalloc((void **)&s->em_e, bt * 2 * sizeof(unsigned long long)));
+    HIPC(hipMalloc((void **)&s->acc_emn, (size_t)Nt * sizeof(double)));
+    HIPC(hipMalloc((void **)&s->acc_eme, (size_t)Nt * sizeof(double)));
+    HIPC(hipMalloc((void **)&s->acc_emh, (size_t)Nt * NR * sizeof(double)));
     s->spec_B = B; s->spec_Nt = Nt; s->spec_NR = NR; s->spec_ND = ND;
     return 0;
+}
+
+// the per-iteration normalised tallies (and the emitted electrons', counted
+// with them) are taken when the caller asks for any of them
+static bool wants_per_iteration(const trk_ctx *c, const trk_tallies *out)
+{
+    const bool em = c->hm.m.work_function > 0.0 && (out->Out_Ne_Em || out->Out_E_Em || out->Out_Ee_vs_E_Em);
+    return em || out->Out_Ee_vs_E || out->Out_Eh_vs_E || out->Out_theta || out->Out_theta_h || out->Out_diff_coeff;
 }
 
 // grow a device array of doubles to hold n values
@@ -396,6 +462,13 @@ void fill_mcdev(const trk_ctx *c, McDev &d)
     d.ln_rhw_el = ts.ln_hw[TRK_TAB_E_EL]; d.ln_rlam_el = ts.ln_lam[TRK_TAB_E_EL];
     d.ln_rhw_hel = ts.ln_hw[TRK_TAB_H_EL]; d.ln_rlam_hel = ts.ln_lam[TRK_TAB_H_EL];
     d.lnES = ts.ln_E[TRK_TAB_SHI]; d.ln_LSsh = ts.ln_L[TRK_TAB_SHI]; d.ln_shi_path = ts.ln_shi_path;
+    d.em_W = d.em_H = d.em_E1 = d.em_gamma = 0.0;
+    const trk_material &m = c->hm.m;
+    if (m.work_function > 0.0 &&
+        trk_barrier_parameters(m.work_function, m.bar_length, m.bar_height, &d.em_E1, &d.em_gamma) == 0) {
+        d.em_W = m.work_function;
+        d.em_H = m.bar_height;
+    }
 }
 
 }  // namespace trk
@@ -638,6 +711,11 @@ static int run_core(trk_ctx *c, McState *s, int64_t nmc, uint64_t seed, int64_t 
     const int ND = h.m.n_dos;
     d.S = Spec{};
     if (want_spec) {
+        HIPC(hipMemsetAsync(s->acc_emn, 0, (size_t)Nt * sizeof(double), st));
+        HIPC(hipMemsetAsync(s->acc_eme, 0, (size_t)Nt * sizeof(double), st));
+        HIPC(hipMemsetAsync(s->acc_emh, 0, (size_t)Nt * NR * sizeof(double), st));
+        k_tenth<<<(NR + 255) / 256, 256, 0, st>>>(s->d_RE, s->d_R, NR);
+        d.RE = s->d_RE; d.em_n = s->em_n; d.em_h = s->em_h; d.em_e = s->em_e;
         HIPC(hipMemsetAsync(s->acc_eve, 0, (size_t)Nt * NR * sizeof(double), st));
         HIPC(hipMemsetAsync(s->acc_ehve, 0, (size_t)Nt * (ND > 1 ? ND : 1) * sizeof(double), st));
         HIPC(hipMemsetAsync(s->acc_th, 0, (size_t)(Nt + 1) * 180 * sizeof(double), st));
@@ -690,6 +768,9 @@ static int run_core(trk_ctx *c, McState *s, int64_t nmc, uint64_t seed, int64_t 
             HIPC(hipMemsetAsync(s->S.th, 0, bt * 180 * sizeof(int), st));
             HIPC(hipMemsetAsync(s->S.thh, 0, bt * 180 * sizeof(int), st));
             HIPC(hipMemsetAsync(s->S.dcs, 0, bt * 2 * sizeof(unsigned long long), st));
+            HIPC(hipMemsetAsync(s->em_n, 0, bt * sizeof(int), st));
+            HIPC(hipMemsetAsync(s->em_h, 0, bt * NR * sizeof(int), st));
+            HIPC(hipMemsetAsync(s->em_e, 0, bt * 2 * sizeof(unsigned long long), st));
             d.S.B = nb;
             d.S.iter0 = (int)(first_iter + b0);
         }
@@ -779,6 +860,13 @@ static int run_core(trk_ctx *c, McState *s, int64_t nmc, uint64_t seed, int64_t 
             k_norm_hist<<<blk((int64_t)Nt * 180), 256, 0, st>>>(s->acc_th, Nt + 1, S.th, S.nel, nb, Nt, 180, 0, nullptr);
             k_norm_hist<<<blk((int64_t)Nt * 180), 256, 0, st>>>(s->acc_thh, Nt + 1, S.thh, S.nvb, nb, Nt, 180, 0, nullptr);
             k_diff_coeff<<<blk(Nt), 256, 0, st>>>(s->acc_dc, S.dcs, S.dcn, nb, Nt);
+            if (d.em_W > 0.0) {
+                k_prefix_i<<<blk(nb), 256, 0, st>>>(s->em_n, nb, Nt, 1);
+                k_prefix_i<<<blk((int64_t)nb * NR), 256, 0, st>>>(s->em_h, nb, Nt, NR);
+                k_prefix_fx128<<<blk(nb), 256, 0, st>>>(s->em_e, nb, Nt);
+                k_norm_hist<<<blk((int64_t)Nt * NR), 256, 0, st>>>(s->acc_emh, Nt, s->em_h, s->em_n, nb, Nt, NR, 1, s->d_RE);
+                k_emit_sums<<<blk(Nt), 256, 0, st>>>(s->acc_emn, s->acc_eme, s->em_n, s->em_e, nb, Nt);
+            }
             HIPC(hipGetLastError());
         }
         if (s->ev1) {
@@ -842,6 +930,11 @@ static int run_core(trk_ctx *c, McState *s, int64_t nmc, uint64_t seed, int64_t 
         if (add_back(out->Out_theta_h, s->acc_thh, (size_t)(Nt + 1) * 180)) return TRK_ERR_DEVICE;
         if (out->Out_diff_coeff)
             HIPC(hipMemcpy(out->Out_diff_coeff, s->acc_dc, (size_t)Nt * sizeof(double), hipMemcpyDeviceToHost));
+        if (d.em_W > 0.0) {
+            if (add_back(out->Out_Ne_Em, s->acc_emn, (size_t)Nt)) return TRK_ERR_DEVICE;
+            if (add_back(out->Out_E_Em, s->acc_eme, (size_t)Nt)) return TRK_ERR_DEVICE;
+            if (add_back(out->Out_Ee_vs_E_Em, s->acc_emh, (size_t)Nt * NR)) return TRK_ERR_DEVICE;
+        }
     }
     c->last_transport_s = transport_s;
     c->last_total_s = std::chrono::duration<double>(std::chrono::steady_clock::now() - tstart).count();
@@ -877,7 +970,8 @@ int run_mc(trk_ctx *c, int64_t nmc, uint64_t seed, int64_t first_iter, const dou
     if (!s->ev0) { HIPC(hipEventCreate(&s->ev0)); HIPC(hipEventCreate(&s->ev1)); }
     if (!s->ncu && device_info(c, s)) return TRK_ERR_DEVICE;
     if (ensure_tally(c, s, Nt, NR, nsh, R)) return TRK_ERR_DEVICE;
-    if (ensure_doubles(c, &s->d_tim, &s->tim_cap, Nt) || ensure_doubles(c, &s->d_R, &s->R_cap, NR))
+    if (ensure_doubles(c, &s->d_tim, &s->tim_cap, Nt) || ensure_doubles(c, &s->d_R, &s->R_cap, NR) ||
+        ensure_doubles(c, &s->d_RE, &s->RE_cap, NR))
         return TRK_ERR_DEVICE;
     const int64_t B = batch_size(c, cap, nmc);
     if (s->cap[K_ION] < B) {
@@ -888,7 +982,7 @@ int run_mc(trk_ctx *c, int64_t nmc, uint64_t seed, int64_t first_iter, const dou
         }
         s->cap[K_ION] = (int)B;
     }
-    const bool want_spec = out->Out_Ee_vs_E || out->Out_Eh_vs_E || out->Out_theta || out->Out_theta_h || out->Out_diff_coeff;
+    const bool want_spec = wants_per_iteration(c, out);
     if (want_spec && ensure_spec(c, s, (int)B, Nt, NR, h.m.n_dos)) return TRK_ERR_DEVICE;
     if (c->trace_host && c->trace_cap > 0) {
         if (s->trace_cap < c->trace_cap) {
@@ -977,6 +1071,11 @@ static size_t stage_doubles(const trk_ctx *c, const trk_tallies *out, int NR, bo
         if (out->Out_Eh_vs_E) n += Nt * ND;
         if (out->Out_theta) n += (Nt + 1) * 180;
         if (out->Out_theta_h) n += (Nt + 1) * 180;
+        if (c->hm.m.work_function > 0.0) {
+            if (out->Out_Ne_Em) n += Nt;
+            if (out->Out_E_Em) n += Nt;
+            if (out->Out_Ee_vs_E_Em) n += plane;
+        }
     }
     return n;
 }
@@ -1022,6 +1121,13 @@ static void carve_state(Carver &cv, McState &s, NoAllocBufs &na, const trk_ctx *
         s.acc_th = cv.take<double>((size_t)(Nt + 1) * 180);
         s.acc_thh = cv.take<double>((size_t)(Nt + 1) * 180);
         s.acc_dc = cv.take<double>((size_t)Nt);
+        s.em_n = cv.take<int>(bt);
+        s.em_h = cv.take<int>(bt * NR);
+        s.em_e = cv.take<unsigned long long>(bt * 2);
+        s.acc_emn = cv.take<double>((size_t)Nt);
+        s.acc_eme = cv.take<double>((size_t)Nt);
+        s.acc_emh = cv.take<double>((size_t)Nt * NR);
+        s.d_RE = cv.take<double>((size_t)NR);
         s.spec_B = B;
     }
     na.stage = cv.take<double>(stage_n);
@@ -1145,11 +1251,15 @@ static int finalize_on_device(trk_ctx *c, McState *s, const NoAllocBufs &na, con
         {out->Out_tot_Ne, (size_t)Nt, &fo.tot_Ne}, {out->Out_tot_Nphot, (size_t)Nt, &fo.tot_Nph},
         {out->Out_tot_E, (size_t)Nt, &fo.tot_E}, {out->Out_E_e, (size_t)Nt, &fo.E_e},
         {out->Out_E_phot, (size_t)Nt, &fo.E_ph}, {out->Out_E_at, (size_t)Nt, &fo.E_at}};
-    double *spec_dev[4] = {nullptr, nullptr, nullptr, nullptr};
+    double *spec_dev[7] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+    const bool em = want_spec && h.m.work_function > 0.0;
     Staged sp[] = {{want_spec ? out->Out_Ee_vs_E : nullptr, plane, &spec_dev[0]},
                    {want_spec && h.m.n_dos > 1 ? out->Out_Eh_vs_E : nullptr, (size_t)Nt * ND, &spec_dev[1]},
                    {want_spec ? out->Out_theta : nullptr, (size_t)(Nt + 1) * 180, &spec_dev[2]},
-                   {want_spec ? out->Out_theta_h : nullptr, (size_t)(Nt + 1) * 180, &spec_dev[3]}};
+                   {want_spec ? out->Out_theta_h : nullptr, (size_t)(Nt + 1) * 180, &spec_dev[3]},
+                   {em ? out->Out_Ne_Em : nullptr, (size_t)Nt, &spec_dev[4]},
+                   {em ? out->Out_E_Em : nullptr, (size_t)Nt, &spec_dev[5]},
+                   {em ? out->Out_Ee_vs_E_Em : nullptr, plane, &spec_dev[6]}};
     size_t off = 0;
     for (Staged *group : {arr, sp}) {
         const size_t cnt = (group == arr) ? sizeof(arr) / sizeof(arr[0]) : sizeof(sp) / sizeof(sp[0]);
@@ -1168,8 +1278,8 @@ static int finalize_on_device(trk_ctx *c, McState *s, const NoAllocBufs &na, con
     k_fin_nodes<<<(unsigned)((Nt + 63) / 64), 64, 0, st>>>(s->T, fo, (int)nat, md.atom_of, md.shl_off,
                                                            h.o.include_photons, na.node_sums);
     k_fin_prefix<<<1, 64, 0, st>>>(fo, Nt, na.node_sums);
-    const double *acc[4] = {s->acc_eve, s->acc_ehve, s->acc_th, s->acc_thh};
-    for (int k = 0; k < 4; ++k)
+    const double *acc[7] = {s->acc_eve, s->acc_ehve, s->acc_th, s->acc_thh, s->acc_emn, s->acc_eme, s->acc_emh};
+    for (int k = 0; k < 7; ++k)
         if (spec_dev[k]) k_add<<<(unsigned)((sp[k].n + 255) / 256), 256, 0, st>>>(spec_dev[k], acc[k], (int64_t)sp[k].n);
     HIPC(hipGetLastError());
     for (Staged *group : {arr, sp}) {
@@ -1198,7 +1308,7 @@ int run_mc_noalloc(trk_ctx *c, int64_t nmc, uint64_t seed, int64_t first_iter, c
     unsigned char *base = nullptr;
     int rc = scratch_base(c, &base);
     if (rc) return rc;
-    const bool want_spec = out->Out_Ee_vs_E || out->Out_Eh_vs_E || out->Out_theta || out->Out_theta_h || out->Out_diff_coeff;
+    const bool want_spec = wants_per_iteration(c, out);
     const size_t stage_n = stage_doubles(c, out, NR, want_spec);
     const size_t limit = (size_t)TRK_NOALLOC_SCRATCH_BYTES;
     McState s{};

@@ -10,8 +10,9 @@
 ! doubles in the order read below; the output file holds the raw doubles of
 ! Out_ne, Out_Ee, Out_nphot, Out_Ephot, Out_Ee_vs_E, Out_Eh_vs_E, Out_Elat,
 ! Out_nh, Out_Eh, Out_Ehkin, Out_tot_Ne, Out_tot_Nphot, Out_tot_E, Out_E_e,
-! Out_E_phot, Out_E_at, Out_E_h, Out_theta, Out_theta_h, Out_diff_coeff in
-! Fortran order, after the error flag and number (two int32).
+! Out_E_phot, Out_E_at, Out_E_h, Out_theta, Out_theta_h, Out_diff_coeff,
+! Out_Ne_Em, Out_E_Em, Out_Ee_vs_E_Em in Fortran order, after the error flag
+! and number (two int32).
 program bridge_driver
    use iso_c_binding
    use Objects
@@ -79,7 +80,8 @@ program bridge_driver
       enddo
    endif
    read(u) Matter%Dens, Matter%At_Dens, Matter%Vsound, Matter%Layer, Matter%N_VB_el, Matter%El_eff_mass, &
-           Matter%hole_mass, Matter%E_F, Matter%v_f, Matter%cut_off, Matter%temp, Matter%work_function
+           Matter%hole_mass, Matter%E_F, Matter%v_f, Matter%cut_off, Matter%temp, Matter%work_function, &
+           Matter%bar_length, Matter%bar_height
    read(u) nd
    allocate(Mat_DOS%E(nd), Mat_DOS%DOS(nd), Mat_DOS%int_DOS(nd), Mat_DOS%k(nd), Mat_DOS%Eff_m(nd))
    allocate(Mat_DOS%k_inv(nd), Mat_DOS%Eff_m_inv(nd))
@@ -130,5 +132,6 @@ program bridge_driver
    write(u) Out_ne, Out_Ee, Out_nphot, Out_Ephot, Out_Ee_vs_E, Out_Eh_vs_E, Out_Elat, Out_nh, Out_Eh, Out_Ehkin
    write(u) Out_tot_Ne, Out_tot_Nphot, Out_tot_E, Out_E_e, Out_E_phot, Out_E_at, Out_E_h, Out_theta, Out_theta_h
    write(u) Out_diff_coeff
+   write(u) Out_Ne_Em, Out_E_Em, Out_Ee_vs_E_Em
    close(u)
 end program bridge_driver

@@ -16,9 +16,9 @@
 ! unnormalised sums into the Out_* arrays, which MAIN then divides by NMC
 ! (MAIN:281-314).  Errors land in Error_message as Save_error_details would
 ! put them (report).  Tables passed in (SHI_MFP, Total_el_MFPs, aidCS, ...)
-! are not read; Out_Eat_dens, Out_theta1, the emission and field arrays are
-! left as they are, as the reference leaves them with emission and fields
-! off.  MPI_param is not used: one process drives one GPU.
+! are not read; Out_Eat_dens, Out_theta1 and the field arrays are left as
+! they are, as the reference leaves them with fields off.  MPI_param is not
+! used: one process drives one GPU.
 module trekis_gpu_bridge
    use iso_c_binding
    use Objects
@@ -135,6 +135,8 @@ contains
       m%cut_off = Matter%cut_off
       m%temp = Matter%temp
       m%work_function = Matter%work_function
+      m%bar_length = Matter%bar_length
+      m%bar_height = Matter%bar_height
       nd = size(Mat_DOS%E)
       m%n_dos = nd
       dos%E = Mat_DOS%E
@@ -263,9 +265,9 @@ contains
       real(8), dimension(:,:), intent(inout), target, contiguous :: Out_theta, Out_theta_h
       real(8), dimension(:), intent(inout) :: Out_theta1
       real(8), dimension(:,:), intent(inout) :: Out_field_all
-      real(8), dimension(:,:), intent(inout) :: Out_Ee_vs_E_Em
-      real(8), dimension(:), intent(inout) :: Out_Ne_Em
-      real(8), dimension(:), intent(inout) :: Out_E_Em
+      real(8), dimension(:,:), intent(inout), target, contiguous :: Out_Ee_vs_E_Em
+      real(8), dimension(:), intent(inout), target, contiguous :: Out_Ne_Em
+      real(8), dimension(:), intent(inout), target, contiguous :: Out_E_Em
       real(8), dimension(:), intent(inout) :: Out_E_field
       type(Error_handling), intent(inout) :: Error_message
       type(Differential_MFP), dimension(:), intent(in) :: DSF_DEMFP, DSF_DEMFP_H
@@ -338,6 +340,9 @@ contains
       t%Out_theta = c_loc(Out_theta)
       t%Out_theta_h = c_loc(Out_theta_h)
       t%Out_diff_coeff = c_loc(Out_diff_coeff)
+      t%Out_Ne_Em = c_loc(Out_Ne_Em)
+      t%Out_E_Em = c_loc(Out_E_Em)
+      t%Out_Ee_vs_E_Em = c_loc(Out_Ee_vs_E_Em)
       t%n_events = 0
       t%n_particles = 0
       rc = trk_run_mc(ctx, int(NMC, c_int64_t), trk_bridge_seed, trk_bridge_first_iter, Out_R, Out_V, t%NR, t)

@@ -31,7 +31,7 @@ module trekis_gpu_c
       real(c_double)     :: ph_A(TRK_MAX_PHONON)
       real(c_double)     :: ph_Gamma(TRK_MAX_PHONON)
       real(c_double)     :: Dens, At_Dens, Vsound, Layer, N_VB_el, El_eff_mass, hole_mass
-      real(c_double)     :: E_F, v_f, cut_off, temp, work_function
+      real(c_double)     :: E_F, v_f, cut_off, temp, work_function, bar_length, bar_height
       integer(c_int32_t) :: n_dos
       type(c_ptr)        :: dos_E, dos_DOS, dos_int_DOS, dos_k, dos_Eff_m, dos_k_inv, dos_Eff_m_inv
    end type trk_material
@@ -58,6 +58,7 @@ module trekis_gpu_c
       type(c_ptr) :: Out_ne, Out_Ee, Out_nphot, Out_Ephot, Out_Ee_vs_E, Out_Eh_vs_E, Out_Elat
       type(c_ptr) :: Out_nh, Out_Eh, Out_Ehkin, Out_tot_Ne, Out_tot_Nphot, Out_tot_E
       type(c_ptr) :: Out_E_e, Out_E_phot, Out_E_at, Out_E_h, Out_theta, Out_theta_h, Out_diff_coeff
+      type(c_ptr) :: Out_Ne_Em, Out_E_Em, Out_Ee_vs_E_Em
       integer(c_int64_t) :: n_events(8)
       integer(c_int64_t) :: n_particles
    end type trk_tallies

@@ -174,8 +174,9 @@ class Context:
 class TallySet:
     """Caller-owned Out_* arrays (Fortran order) plus the C struct pointing at them."""
 
-    FIELDS_NT_NR = ["Out_ne", "Out_Ee", "Out_nphot", "Out_Ephot", "Out_Ee_vs_E", "Out_Elat"]
-    FIELDS_NT = ["Out_tot_Ne", "Out_tot_Nphot", "Out_tot_E", "Out_E_e", "Out_E_phot", "Out_E_at", "Out_diff_coeff"]
+    FIELDS_NT_NR = ["Out_ne", "Out_Ee", "Out_nphot", "Out_Ephot", "Out_Ee_vs_E", "Out_Elat", "Out_Ee_vs_E_Em"]
+    FIELDS_NT = ["Out_tot_Ne", "Out_tot_Nphot", "Out_tot_E", "Out_E_e", "Out_E_phot", "Out_E_at", "Out_diff_coeff",
+                 "Out_Ne_Em", "Out_E_Em"]
 
     def __init__(self, Nt, NR, desc):
         na = desc.n_atoms

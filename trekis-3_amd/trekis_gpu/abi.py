@@ -57,6 +57,8 @@ class Material(ctypes.Structure):
         ("cut_off", c_double),
         ("temp", c_double),
         ("work_function", c_double),
+        ("bar_length", c_double),
+        ("bar_height", c_double),
         ("n_dos", c_int32),
         ("dos_E", DP),
         ("dos_DOS", DP),
@@ -118,6 +120,8 @@ class Tallies(ctypes.Structure):
         ("Out_E_h", DP),
         ("Out_theta", DP), ("Out_theta_h", DP),
         ("Out_diff_coeff", DP),
+        ("Out_Ne_Em", DP), ("Out_E_Em", DP),
+        ("Out_Ee_vs_E_Em", DP),
         ("n_events", c_int64 * 8),
         ("n_particles", c_int64),
     ]
@@ -157,6 +161,7 @@ def declare_library(lib):
         "trk_check_atomic_parameters": (c_int, [P(Material), c_int32, c_char_p, c_size_t]),
         "trk_sumrules": (c_int, [DP, DP, DP, c_int32, c_double, c_double, DP, DP]),
         "trk_w_plasma": (c_double, [c_double, c_double]),
+        "trk_barrier_parameters": (c_int, [c_double, c_double, c_double, DP, DP]),
         "trk_dos_size": (c_int, [DP, c_int32, P(c_int32)]),
         "trk_prepare_dos": (c_int, [DP, DP, c_int32, c_double, c_double, c_double,
                                     DP, DP, DP, DP, DP, DP, DP, c_int32]),

@@ -118,7 +118,8 @@ class Batch:
         out = []
         for c in self.impacts:
             t = TallySet(c.Nt, len(self.R), c.desc)
-            for k in ("Out_Ee_vs_E", "Out_Eh_vs_E", "Out_theta", "Out_theta_h", "Out_diff_coeff"):
+            for k in ("Out_Ee_vs_E", "Out_Eh_vs_E", "Out_theta", "Out_theta_h", "Out_diff_coeff",
+                      "Out_Ne_Em", "Out_E_Em", "Out_Ee_vs_E_Em"):
                 setattr(t.s, k, None)
             out.append(t)
         return out

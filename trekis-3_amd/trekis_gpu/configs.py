@@ -62,6 +62,9 @@ class MaterialDesc:
     hole_mass: float = 1.0
     cut_off: float = 10.0
     temp: float = 0.0
+    work_function: float = 0.0                  # [eV] > 0: emission through the surface Z = 0
+    bar_length: float = 0.0                     # [A] surface barrier length
+    bar_height: float = 0.0                     # [eV] surface barrier height
     phonon: List[Tuple[float, float, float]] = field(default_factory=list)
     phonon_single_pole: bool = False
     dos_raw_E: np.ndarray = None
@@ -186,7 +189,9 @@ class MaterialDesc:
         m.v_f = self.v_f
         m.cut_off = self.cut_off
         m.temp = self.temp
-        m.work_function = getattr(self, "work_function", 0.0)   # the library refuses emission (> 0)
+        m.work_function = self.work_function
+        m.bar_length = self.bar_length
+        m.bar_height = self.bar_height
         m.n_dos = self.n_dos
         for k, f in [("E", "dos_E"), ("DOS", "dos_DOS"), ("int_DOS", "dos_int_DOS"), ("k", "dos_k"),
                      ("Eff_m", "dos_Eff_m"), ("k_inv", "dos_k_inv"), ("Eff_m_inv", "dos_Eff_m_inv")]:
@@ -410,6 +415,8 @@ def config_material(name):
         mat.temp = temp
         ion.Kind_Zeff, ion.fixed_Zeff = kz, fz
         ion.Kind_ion = rest[0] if rest else 0
+        if len(rest) > 1:
+            mat.work_function, mat.bar_length, mat.bar_height = rest[1]
         opt.kind_of_DR, opt.plasmon_Emax = dr, plasmon
         opt.dt_flag, opt.dt = dt_flag, dt
         return mat, ion, opt, run
@@ -436,12 +443,18 @@ def config_material(name):
 #   opt_bk_ion        a Brandt-Kitagawa ion (Kind_ion = 1, a 9th entry): its
 #                     mean free paths with the charge form factor inside the q
 #                     integral (SHI_TotIMFP_BK, CS:2748-2889), Bohr charge
+#   opt_emission      emission through the surface Z = 0 (a 10th entry: work
+#                     function [eV], barrier length [A], barrier height [eV];
+#                     calculate_emission, MC:2477-2513): electrons above 15 eV
+#                     leave, slower ones through the barrier's transmission or
+#                     are turned back; the ion's track starts at the surface
 OPTION_WORKLOADS = {
     "opt_plasmon_pole": ("test", 2, 1, 8.0, 0.0, 1, 1, 2.0),
     "opt_ritchie_warm": ("test", 3, 2, 8.0, 300.0, 0, 0, 1.0),
     "opt_sg_warm": ("test", 1, 3, 8.0, 300.0, 0, 0, 1.0),
     "opt_fixed_zeff": ("test_metal", 1, 4, 3.5, 300.0, 0, 1, 10.0),
     "opt_bk_ion": ("test", 1, 1, 8.0, 0.0, 0, 0, 1.0, 1),
+    "opt_emission": ("test", 1, 0, 8.0, 0.0, 0, 0, 1.0, 0, (4.5, 10.0, 10.0)),
 }
 
 

@@ -316,7 +316,9 @@ def parse_parameters(text: str) -> Parameters:
     (ph,) = read("i")
     p["include_photons"] = int(ph == 1)
     wf, bl, bh = read("rrr")
+    # RF:378-381: emission is on only when all three are positive
     p["work_function"] = wf if (wf > 0.0 and bl > 0.0 and bh > 0.0) else 0.0
+    p["bar_length"], p["bar_height"] = bl, bh
     (p["NMC"],) = read("i")
     (p["Num_th"],) = read("i")
     p["DOS_file"] = p["CDF_file"] = None
@@ -331,9 +333,9 @@ def load_inputs(parameters_path: str, root: str = None, radiat=None):
     """(MaterialDesc, Ion, Options, NMC) from the reference's input tree: the
     parameter file and, under root (default: the parameter file's folder),
     INPUT_CDF/<material>.cdf and INPUT_DOS/<material>.dos (or the files the
-    parameter file names).  Options this library does not compute (surface
-    emission, Brandt-Kitagawa ions, DSF elastic tables) are refused by the
-    library when the context is created.
+    parameter file names).  Options this library does not compute (DSF
+    elastic tables, Delta-CDF, BEB) are refused by the library when the
+    context is created.
 
     radiat: radiative decay times [fs] per (element, shell), which the
     reference reads from the EADL database (Dealing_with_EADL.f90:356).  With
@@ -352,6 +354,8 @@ def load_inputs(parameters_path: str, root: str = None, radiat=None):
                    el_eff_mass=p["El_eff_mass"], radiat=None if isinstance(radiat, str) else radiat)
     mat.temp = p["temp"]
     mat.work_function = p["work_function"]
+    if mat.work_function > 0.0:
+        mat.bar_length, mat.bar_height = p["bar_length"], p["bar_height"]
     ion = abi.Ion()
     ion.Zat, ion.Mass, ion.E = p["Z"], p["mass"], p["E_MeV"] * 1.0e6
     ion.Kind_Zeff, ion.fixed_Zeff, ion.Kind_ion = p["Kind_Zeff"], p["fixed_Zeff"], p["Kind_ion"]

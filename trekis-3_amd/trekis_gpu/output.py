@@ -19,13 +19,24 @@ G_KB = 11604.0
 
 PER_ITERATION = ["Out_tot_Ne", "Out_tot_Nphot", "Out_tot_E", "Out_E_e", "Out_E_phot", "Out_E_at", "Out_E_h",
                  "Out_ne", "Out_Ee", "Out_nphot", "Out_Ephot", "Out_Ee_vs_E", "Out_Eh_vs_E", "Out_Elat",
-                 "Out_theta", "Out_theta_h", "Out_diff_coeff", "Out_nh", "Out_Eh", "Out_Ehkin"]
+                 "Out_theta", "Out_theta_h", "Out_diff_coeff", "Out_nh", "Out_Eh", "Out_Ehkin",
+                 "Out_Ne_Em", "Out_E_Em", "Out_Ee_vs_E_Em"]
 
 
 def normalise(tallies, nmc):
-    """Every Out_* array divided by dble(NMC) (MAIN:281-314)."""
+    """Every Out_* array divided by dble(NMC) (MAIN:281-314); the emitted
+    electrons' arrays (MAIN:301-303) are zeros when the tallies have none."""
     n = float(nmc)
-    return {k: tallies[k] / n for k in PER_ITERATION}
+    out = {}
+    for k in PER_ITERATION:
+        try:
+            out[k] = tallies[k] / n
+        except KeyError:
+            if not k.endswith("_Em"):
+                raise
+            shape = np.shape(tallies["Out_Ee_vs_E"] if k == "Out_Ee_vs_E_Em" else tallies["Out_tot_Ne"])
+            out[k] = np.zeros(shape)
+    return out
 
 
 def _cum_lattice(Elat):
@@ -58,6 +69,7 @@ def radial_tables(norm, mat, vb=None):
     t["Radial_photon_energy[eV_A^-3].txt"] = norm["Out_Ephot"].T
     t["Electron_distribution_vs_E[1_eV].txt"] = norm["Out_Ee_vs_E"].T
     t["VB_holes_distribution_vs_E[1_eV].txt"] = norm["Out_Eh_vs_E"].T
+    t["Emitted_electron_distribution_vs_E[1_eV].txt"] = norm["Out_Ee_vs_E_Em"].T
     lat = _cum_lattice(norm["Out_Elat"])
     t["Radial_Lattice_energy[eV_A^-3].txt"] = lat.T
     a, s = vb if vb is not None else (0, len(mat.atoms[0].shells) - 1)
@@ -99,7 +111,8 @@ def _shell_tag(atom, shell):
     return shell_name(shell.shl_num)
 
 
-def write_output(folder, tallies, nmc, mat, time_grid, Out_R, include_photons, dos_E=None, hole_mass=None):
+def write_output(folder, tallies, nmc, mat, time_grid, Out_R, include_photons, dos_E=None, hole_mass=None,
+                 work_function=None):
     """Write the result files of Save_output (Sorting_output_data.f90:340-1140)
     into folder, from a run's unnormalised tallies (divided by nmc here, as
     MAIN:281-314 does): the angular distributions, Total_numbers.txt,
@@ -107,10 +120,11 @@ def write_output(folder, tallies, nmc, mat, time_grid, Out_R, include_photons, d
     electron / photon / lattice / track files, the energy spectra and every
     shell's hole files, with the reference's names, headers and Fortran
     fields ('(f10.2)' time headers, '(f9.1)' row labels, '(e)' values as
-    E25.16).  Quantities of the emission and field models, which this
-    library does not build, are written as zeros where the files hold them.
-    The parameter echo '!Parameters.txt' is not written.  Returns the file
-    names."""
+    E25.16); with emission on (work_function > 0, default the material's)
+    also the emitted electrons' spectrum.  Quantities of the field model,
+    which this library does not build, are written as zeros where the files
+    hold them.  The parameter echo '!Parameters.txt' is not written.
+    Returns the file names."""
     import os
     os.makedirs(folder, exist_ok=True)
     n = normalise(tallies, nmc)
@@ -118,6 +132,7 @@ def write_output(folder, tallies, nmc, mat, time_grid, Out_R, include_photons, d
     times = np.asarray(time_grid, dtype=float)
     R = np.asarray(Out_R, dtype=float)
     hole_mass = mat.hole_mass if hole_mass is None else hole_mass
+    work_function = getattr(mat, "work_function", 0.0) if work_function is None else work_function
     written = []
 
     def path(name):
@@ -132,18 +147,22 @@ def write_output(folder, tallies, nmc, mat, time_grid, Out_R, include_photons, d
             f.write(_time_header(first, times))
             for i in range(180):
                 f.write(_e(float(i + 1)) + "".join(_e(th[k, i]) for k in range(N)) + " \n")
-    zero = np.zeros(N)
+    # emitted electrons' spectrum (SO:592-615), rows labelled by Out_R/10
+    if work_function > 0.0:
+        _matrix_file(path("Emitted_electron_distribution_vs_E[1_eV].txt"), "#Energy[eV] ", times, R / 10.0,
+                     radial_tables(n, mat)["Emitted_electron_distribution_vs_E[1_eV].txt"][:, :N])
     # totals (SO:618-640)
+    Nem, Eem = n["Out_Ne_Em"], n["Out_E_Em"]
     with open(path("Total_numbers.txt"), "w") as f:
         if include_photons:
             f.write("#Time[fs]    Ne    Ne_Emitted    Energy[eV]     Energy_Emitted[eV] N_photons\n")
             for k in range(N):
-                f.write("".join(_e(v) for v in (times[k], n["Out_tot_Ne"][k], zero[k], n["Out_tot_E"][k], zero[k],
+                f.write("".join(_e(v) for v in (times[k], n["Out_tot_Ne"][k], Nem[k], n["Out_tot_E"][k], Eem[k],
                                                  n["Out_tot_Nphot"][k])) + "\n")
         else:
             f.write("#Time[fs]    Ne    Ne_Emitted    Energy[eV]     Energy_Emitted[eV]\n")
             for k in range(N):
-                f.write("".join(_e(v) for v in (times[k], n["Out_tot_Ne"][k], zero[k], n["Out_tot_E"][k], zero[k]))
+                f.write("".join(_e(v) for v in (times[k], n["Out_tot_Ne"][k], Nem[k], n["Out_tot_E"][k], Eem[k]))
                         + "\n")
     # hole diffusion coefficient (SO:646-662): the file stays empty without hole transport
     with open(path("Hole_mean_diffusion_coefficient.txt"), "w") as f:
