@@ -113,7 +113,10 @@ typedef struct trk_ion {
 typedef struct trk_options {
     int32_t kind_of_EMFP;               /* -1 no elastic, 0 Mott, 1 phonon CDF */
     int32_t CDF_elast_Zeff;             /* 0 Barkas-like target charge, 1 unit charge */
-    int32_t kind_of_DR;                 /* 1 free electron, 2 plasmon pole, 3 Ritchie */
+    int32_t kind_of_DR;                 /* 1 free electron, 2 plasmon pole, 3 Ritchie, 4 Delta-CDF (the
+                                           electron and hole inelastic cross sections of delta peaks,
+                                           Cross_sections.f90:952-956, 1894-1896; loss functions elsewhere
+                                           disperse as 1) */
     int32_t plasmon_Emax;               /* use plasmon energy as integration limit */
     int32_t include_photons;            /* radiative decay and photon transport */
     int32_t dt_flag;                    /* 0 linear, 1 logarithmic time grid */
@@ -351,6 +354,11 @@ int  trk_check_atomic_parameters(trk_material *m, int32_t include_photons, char 
  * when the parameters give no finite E1 and gamma. */
 int  trk_barrier_parameters(double work_function, double bar_length, double bar_height,
                             double *E1, double *gamma);
+
+/* Weight of an oscillator's delta peak in the Delta-CDF model
+ * (kind_of_DR = 4): its k-sum integral above x_min [eV]
+ * (define_alpha, Reading_files_and_parameters.f90:2199-2239). */
+double trk_delta_alpha(double A, double E0, double Gamma, double x_min);
 
 /* f-sum and k-sum rules of a set of Drude-Lorentz oscillators
  * (sumrules, Cross_sections.f90:728-749). */

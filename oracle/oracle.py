@@ -61,6 +61,10 @@ def load():
         "or_time_nodes": (c_int, [c_void_p, P(c_int32), DP, c_int32]),
         "or_constant": (c_double, [c_int]),
         "or_barrier_parameters": (c_int, [c_double, c_double, c_double, DP, DP]),
+        "or_delta_alpha": (c_double, [c_double, c_double, c_double, c_double]),
+        "or_delta_cs": (c_double, [c_void_p, c_int, c_double, c_double, c_int, c_double]),
+        "or_delta_eloss": (c_double, [c_void_p, c_int, c_double, c_double]),
+        "or_delta_transfer": (c_double, [c_void_p, c_int, c_double, c_double, c_double]),
         "or_sample_transfer": (c_double, [c_void_p, c_int, c_int, c_double, c_double]),
         "or_mott_theta": (c_double, [c_void_p, c_int, c_double, c_double, DP]),
         "or_mott_W": (c_double, [c_void_p, c_int, c_double, c_double, c_double]),

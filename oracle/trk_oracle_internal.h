@@ -71,6 +71,8 @@ struct or_ctx {
     /* surface barrier (emission when em_W > 0): work function, height and
      * the logistic transmission's E1 and gamma */
     double em_W, em_H, em_E1, em_gamma;
+    /* Delta-CDF weight of every oscillator (kind_of_DR = 4) */
+    double alpha[TRK_MAX_OSC];
     /* time grid */
     int32_t Nt;
     double *tim_glob;
@@ -82,6 +84,12 @@ struct or_ctx {
 double or_imewq(const or_ctx *c, double hw, double hq, int shell, int photon);
 double or_shi_zeff(const or_ctx *c, double E);
 double or_rest_energy(double M0);
+/* the Delta-CDF model (trk_oracle_delta.c) */
+double or_delta_alpha(double A, double E0, double Gamma, double x_min);
+double or_delta_cs(const or_ctx *c, int shell, double E, double Ip, int has_emax, double Emax_in);
+double or_delta_eloss(const or_ctx *c, int shell, double E, double Ip);
+void   or_delta_tot_imfp(const or_ctx *c, int shell, double Ele, double Emin, double *L, double *dEdx);
+double or_delta_transfer(const or_ctx *c, int shell, double Ee, double Ip, double RN);
 double or_nfp_1d(or_ctx *c, double E, const double *Earr, const double *Larr, int32_t n);
 double or_nfp_2d(or_ctx *c, double E, const double *Earr, const double *Larr, int32_t n);
 double or_interp_transfer(or_ctx *c, const or_table *t, int shell, double Ele, double L_need);

@@ -478,6 +478,9 @@ static void tot_imfp(const or_ctx *c, double Ele, int shell, int kind, double *S
         *Sigma = 1.0 / (Mass * Ltot1);
         *dEdx = Mass * ddEdx;
     }
+    /* the Delta-CDF model's closed forms in place of the quadrature's totals
+     * (CS:952-956); the rows stay the quadrature's */
+    if (c->o.kind_of_DR == 4 && kind <= 1) or_delta_tot_imfp(c, shell, Ele, Emin, Sigma, dEdx);
     if (row) {
         for (int k = 0; k < cap; ++k) {
             if (cum[k] < 1.0e-15) cum[k] = 1.0e15;
@@ -1138,6 +1141,12 @@ or_ctx *or_create(const trk_material *mat, const trk_ion *ion, const trk_options
     }
     c->shl_off[mat->n_atoms] = s;
     c->osc_off[s] = o;
+    /* Delta-CDF weights: each oscillator's k-sum above its shell's edge
+     * (define_alpha, RF:1385-1388, 1572-1574) */
+    if (opt->kind_of_DR == 4)
+        for (int sh = 0; sh < s; ++sh)
+            for (int k = c->osc_off[sh]; k < c->osc_off[sh + 1]; ++k)
+                c->alpha[k] = or_delta_alpha(mat->osc_A[k], mat->osc_E0[k], mat->osc_Gamma[k], mat->Ip[sh]);
     c->nsh = s;
     c->vb = mat->n_shells[0] - 1;
     c->Egap = mat->Ip[c->vb];

@@ -173,3 +173,18 @@ def test_batch_autotuner_records_its_choice():
     assert (cfg["block"], cfg["round_events"]) == (best["block"], best["round_events"])
     t = b.run(0x5EED0042, 9000)
     assert sum(float(x["Out_tot_Ne"].sum()) for x in t) > 0
+
+
+def test_batch_refuses_the_delta_cdf_model():
+    """Impacts whose material uses the Delta-CDF model (kind_of_DR = 4) run
+    through trk_run_mc; a batch refuses them with the reason."""
+    lib = tg.load_library()
+    mat, ion, opt, run = tg.config_material("opt_delta_cdf")
+    mat.prepare(lib)
+    ctx = tg.Context(mat, ion, opt)
+    ctx.build_tables()
+    c = IonContext(ctx, ion)
+    c.build_tables()
+    R, V = tg.radial_grid(run["NR"], run["Rmin"], run["Rmax"], 100.0)
+    with pytest.raises(tg.TrekisError, match="Delta-CDF"):
+        Batch([c], R, V, capacity=1 << 16)

@@ -22,7 +22,7 @@ import pytest
 import trekis_gpu as tg
 from oracle import Oracle, dp
 
-CONFIGS = ["test", "test_metal", "opt_emission"]
+CONFIGS = ["test", "test_metal", "opt_emission", "opt_delta_cdf"]
 
 
 @pytest.fixture(scope="module", params=CONFIGS)
@@ -32,8 +32,8 @@ def runs(request, lib):
     o = Oracle(mat, ion, opt)
     o.build_tables(8, shi_window=(ion.E * 0.8, ion.E * 1.01))
     R, V = tg.radial_grid(run["NR"], run["Rmin"], run["Rmax"], mat.layer)
-    a = o.run_mc(run["nmc"], run["seed"], R, V, tg.TallySet(o.Nt, len(R), mat), trace_cap=2_000_000)
-    b = o.run_mc(run["nmc"], run["seed"], R, V, tg.TallySet(o.Nt, len(R), mat), trace_cap=2_000_000)
+    a = o.run_mc(run["nmc"], run["seed"], R, V, tg.TallySet(o.Nt, len(R), mat), trace_cap=3_000_000)
+    b = o.run_mc(run["nmc"], run["seed"], R, V, tg.TallySet(o.Nt, len(R), mat), trace_cap=3_000_000)
     tim = np.zeros(o.Nt)
     n = ctypes.c_int32()
     o.lib.or_time_nodes(o.h, ctypes.byref(n), dp(tim), o.Nt)

@@ -30,6 +30,16 @@ the reference's expressions:
 * the valence-hole gap walk (check_hole_parameters, MC:696-719): brute
   force over a small DOS with empty levels -- the nearest nonempty level
   below (hole-electron) or above (hole-lattice) the bracket of the energy.
+* the Delta-CDF model (kind_of_DR = 4; define_alpha RF:2199-2239,
+  Integral_CDF_delta_CS / energy_loss_delta CS:1449-1786,
+  get_inelastic_energy_transfer CS:2051-2123): a peak's weight is its
+  oscillator's k-sum above the edge (numerical integral); well above the
+  peaks the cross section's derivative in the transferred energy is
+  1/(pi a0 n m v^2) times sum alpha_i / (W (W - E0_i)); the stopping power is
+  the W-weighted integral of that cross section; with one peak per shell the
+  linear extrapolation meets the model at its switching energy and vanishes
+  at the edge; the sampled transfer's integrated cross section is the drawn
+  fraction of the total.
 * barrier_parameters (MC:2094-2116): the logistic transmission
   1/(1 + exp(gamma (E1 - E))) of the surface is fitted to Eckart's
   transmission through a barrier of height H, work function W and length L
@@ -340,3 +350,104 @@ def test_surface_barrier_is_the_eckart_midpoint(lib):
         E1, g = ctypes.c_double(), ctypes.c_double()
         assert olib.or_barrier_parameters(W, L, H, ctypes.byref(E1), ctypes.byref(g)) != 0
         assert lib.trk_barrier_parameters(W, L, H, ctypes.byref(E1), ctypes.byref(g)) != 0
+
+
+def _delta_oracle(lib, single_peak=False):
+    import copy
+    mat, ion, opt, run = tg.config_material("opt_delta_cdf")
+    if single_peak:
+        mat = copy.deepcopy(mat)
+        for a in mat.atoms:
+            for sh in a.shells:
+                sh.osc = sh.osc[:1]
+    mat.prepare(lib)
+    return mat, Oracle(mat, ion, opt)
+
+
+def test_delta_cdf_weights_are_the_k_sum(lib):
+    """alpha = integral over x > x_min of x Im[-1/eps] = A G x^2 / ((x^2 - E0^2)^2
+    + G^2 x^2) (numerical quadrature); the library's host copy is the
+    oracle's bit for bit."""
+    import oracle as orc
+    olib = orc.load()
+    for A, E0, G, xm in ((0.5, 20.0, 10.0, 5.0), (1.2, 15.0, 3.0, 9.0), (0.1, 300.0, 50.0, 280.0),
+                         (0.8, 10.0, 25.0, 0.1), (2.0, 40.0, 90.0, 12.0)):
+        a = olib.or_delta_alpha(A, E0, G, xm)
+        assert lib.trk_delta_alpha(A, E0, G, xm) == a
+        f = lambda x: A * G * x * x / ((x * x - E0 * E0) ** 2 + G * G * x * x)
+        q = integrate.quad(f, xm, np.inf, epsabs=0, epsrel=1e-12, limit=500)[0]
+        assert a == pytest.approx(q, rel=1e-12), (A, E0, G, xm)
+
+
+def _delta_peaks(o, mat, s):
+    E0 = [e for a in mat.atoms for sh in a.shells for (e, _, _) in sh.osc]
+    off = np.cumsum([0] + [len(sh.osc) for a in mat.atoms for sh in a.shells])
+    A = [o.mat.osc_A[k] for k in range(off[s], off[s + 1])]
+    G = [o.mat.osc_Gamma[k] for k in range(off[s], off[s + 1])]
+    Ip = [sh.Ip for a in mat.atoms for sh in a.shells][s]
+    import oracle as orc
+    olib = orc.load()
+    al = [olib.or_delta_alpha(A[i], E0[off[s] + i], G[i], Ip) for i in range(len(A))]
+    return np.array(E0[off[s]:off[s + 1]]), np.array(al), max(Ip, 1e-3)
+
+
+def test_delta_cdf_cross_section_and_stopping(lib):
+    """Well above the peaks, d sigma / dW of the closed form (sigma integrated
+    up to W, differentiated numerically) is P sum alpha_i / (W (W - E0_i)),
+    P = 1e24 / (pi a0 n m c^2 beta^2), to 1e-3 (the relativistic terms are of
+    order W / m c^2).  Integrating that nonrelativistic form over the
+    transfers [max(Ip, E0 (1 - E0/4E)), (E + Ip)/2] (principal values at the
+    peaks) gives the total cross section, up to its sign (the reference keeps
+    |sigma|), to 1e-3 and, weighted with W, the stopping power to 3e-3 up to
+    20 keV."""
+    mat, o = _delta_oracle(lib)
+    n = mat.At_Dens
+    for s, E in ((1, 1.0e3), (1, 1.0e4), (0, 4.0e3), (0, 2.0e4), (2, 5.0e3), (2, 2.0e4)):
+        E0, al, Ip = _delta_peaks(o, mat, s)
+        gam = 1.0 + E / MEC2
+        beta2 = 1.0 - 1.0 / (gam * gam)
+        P = 1e24 / (math.pi * A0 * n * MEC2 * beta2)
+        Wmax = (E + Ip) / 2.0
+        for W in np.geomspace(max(3.0 * E0.max(), 2 * Ip), Wmax / 3.0, 5):
+            h = W * 1e-5
+            d = (o.lib.or_delta_cs(o.h, s, E, Ip, 1, W + h) - o.lib.or_delta_cs(o.h, s, E, Ip, 1, W - h)) / (2 * h)
+            want = P * float(np.sum(al / (W * (W - E0))))
+            assert d == pytest.approx(want, rel=1e-3), (s, E, W)
+        Wmin = np.maximum(Ip, E0 * (1.0 - E0 / (4.0 * E)))
+        pole = np.log(np.abs((Wmax - E0) / (Wmin - E0)))
+        sig_nr = P * float(np.sum(al / E0 * (np.log(Wmax / Wmin) - pole)))
+        assert o.lib.or_delta_cs(o.h, s, E, Ip, 0, 0.0) == pytest.approx(abs(sig_nr), rel=1e-3), (s, E)
+        S_nr = n * 1e-24 * P * float(np.sum(al * pole))
+        assert o.lib.or_delta_eloss(o.h, s, E, Ip) == pytest.approx(S_nr, rel=3e-3), (s, E)
+    o.close()
+
+
+def test_delta_cdf_linear_branch_and_sampling(lib):
+    """One peak per shell: the cross section is 0 at the edge, linear up to the
+    switching energy Eeq (1 + 1/100) and continuous with the model there; the
+    switching energy lies above the energy where the model's transfer window
+    [max(Ip, E0 (1 - E0/4E)), (E + Ip)/2] opens.  A sampled transfer's
+    integrated cross section is RN times the total to the search's 1e-3."""
+    mat, o = _delta_oracle(lib, single_peak=True)
+    for s in range(sum(mat.n_shells)):
+        E0, al, Ip = _delta_peaks(o, mat, s)
+        e0 = float(E0[0])
+        Eeq = 5.0 / 4.0 * e0 - Ip / 2.0 + 0.25 * math.sqrt(17.0 * e0 * e0 - 12.0 * Ip * e0 + 4.0 * Ip * Ip)
+        sw = Eeq + Eeq / 100.0
+        opens = [x for x in np.linspace(Ip, sw, 20001) if (x + Ip) / 2 >= max(Ip, e0 * (1 - e0 / (4 * x)))]
+        assert opens and opens[0] <= Eeq
+        assert o.lib.or_delta_cs(o.h, s, Ip, Ip, 0, 0.0) == 0.0
+        lin = [o.lib.or_delta_cs(o.h, s, Ip + f * (sw - Ip), Ip, 0, 0.0) for f in (0.25, 0.5, 1.0)]
+        assert lin[1] == pytest.approx(2 * lin[0], rel=1e-9) and lin[2] == pytest.approx(4 * lin[0], rel=1e-9)
+        above = o.lib.or_delta_cs(o.h, s, sw * (1 + 1e-9), Ip, 0, 0.0)
+        assert above == pytest.approx(lin[2], rel=1e-6), (s, above, lin[2])
+        for E in (sw * 1.5, 10 * sw, 1e4):
+            tot = o.lib.or_delta_cs(o.h, s, E, Ip, 1, (Ip + E) * 0.5)
+            for RN in (0.05, 0.3, 0.77, 0.99):
+                dE = o.lib.or_delta_transfer(o.h, s, E, Ip, RN)
+                assert Ip <= dE <= (Ip + E) * 0.5
+                got = o.lib.or_delta_cs(o.h, s, E, Ip, 1, dE)
+                ok = abs(got - RN * tot) <= 1e-3 * RN * tot
+                lo, hi = o.lib.or_delta_cs(o.h, s, E, Ip, 1, dE - 1e-3), o.lib.or_delta_cs(o.h, s, E, Ip, 1, dE + 1e-3)
+                assert ok or lo <= RN * tot <= hi, (s, E, RN, dE, got / tot)
+    o.close()

@@ -70,8 +70,8 @@ static int validate(const trk_material *m, const trk_ion *ion, const trk_options
     if (ion->Kind_ion != 0 && ion->Kind_ion != 1) { why = "Kind_ion must be 0 (point charge) or 1 (Brandt-Kitagawa)"; return 1; }
     if (!(o->Tim > 0.0) || !(o->dt > 0.0)) { why = "Tim and dt must be positive"; return 1; }
     if (o->dt_flag >= 1 && o->dt < 2.0) { why = "a logarithmic time grid needs dt >= 2 (RF:299)"; return 1; }
-    // kind 4 is the Delta-CDF model in the reference's TotIMFP (CS:955), which is not built
-    if (o->kind_of_DR < 1 || o->kind_of_DR > 3) { why = "kind_of_DR must be 1, 2 or 3 (4, Delta-CDF, is not built)"; return 1; }
+    if (o->kind_of_DR < 1 || o->kind_of_DR > 4)
+        { why = "kind_of_DR must be 1 (free electron), 2 (plasmon pole), 3 (Ritchie) or 4 (Delta-CDF)"; return 1; }
     if (o->precision != 0 && o->precision != 1) { why = "precision must be 0 (FP64) or 1 (mixed)"; return 1; }
     if (o->quadrature != 0 && o->quadrature != 1) { why = "quadrature must be 0 (reference, FP64) or 1 (FP32 compensated)"; return 1; }
     if (o->quadrature == 1 && !(m->El_eff_mass > 0.0))
@@ -148,6 +148,15 @@ int trk_ctx_create(const trk_material *mat, const trk_ion *ion, const trk_option
     rc |= upload(c, &b.oA, mat->osc_A, (size_t)o);
     rc |= upload(c, &b.oE0, mat->osc_E0, (size_t)o);
     rc |= upload(c, &b.oG, mat->osc_Gamma, (size_t)o);
+    if (opt->kind_of_DR == 4) {
+        // the delta peaks' weights, each oscillator's k-sum above its
+        // shell's edge (define_alpha, RF:1385-1388, 1572-1574)
+        std::vector<double> alpha((size_t)o);
+        for (int sh = 0; sh < s; ++sh)
+            for (int k = h.osc_off[sh]; k < h.osc_off[sh + 1]; ++k)
+                alpha[k] = trk_delta_alpha(mat->osc_A[k], mat->osc_E0[k], mat->osc_Gamma[k], mat->Ip[sh]);
+        rc |= upload(c, &b.oAlpha, alpha.data(), alpha.size());
+    }
     rc |= upload(c, &b.phA, mat->ph_A, (size_t)mat->n_phonon);
     rc |= upload(c, &b.phE0, mat->ph_E0, (size_t)mat->n_phonon);
     rc |= upload(c, &b.phG, mat->ph_Gamma, (size_t)mat->n_phonon);
@@ -210,7 +219,7 @@ void trk_ctx_destroy(trk_ctx *c)
     mc_free(c->mc);
     MatBuffers &b = c->mb;
     void *ps[] = {b.osc_off, b.atom_of, b.shl_off, b.Zat, b.oA, b.oE0, b.oG, b.phA, b.phE0, b.phG, b.k, b.Effm,
-                  b.dosE, b.dos, b.dosI, b.dosM, b.Ip, b.Nel, b.Auger, b.Radiat, b.Mass, b.Pers, b.mott};
+                  b.dosE, b.dos, b.dosI, b.dosM, b.Ip, b.Nel, b.Auger, b.Radiat, b.Mass, b.Pers, b.mott, b.oAlpha};
     for (void *p : ps) if (p) (void)hipFree(p);
     for (hipEvent_t e : c->pe) if (e) (void)hipEventDestroy(e);
     if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -231,7 +240,7 @@ int trk_build_tables(trk_ctx *c)
     std::string err;
     if (c->base && !c->base->tables_ready)
         return set_error(c, TRK_ERR_STATE, "the material context's tables are not built");
-    int rc = build_tables(c->hm, c->md, c->ts, c->stream, err, c->base ? &c->base->ts : nullptr);
+    int rc = build_tables(c->hm, c->md, c->ts, c->stream, err, c->base ? &c->base->ts : nullptr, c->mb.oAlpha);
     if (rc) return set_error(c, rc, "table build failed: %s", err.c_str());
     c->tables_ready = true;
     return 0;

@@ -35,6 +35,7 @@ struct MatBuffers {
     double *Ip = nullptr, *Nel = nullptr, *Auger = nullptr, *Radiat = nullptr;
     double *Mass = nullptr, *Pers = nullptr;
     double *mott = nullptr;     // [4][n_atoms]: Z, Z^(2/3), rest energy, Pers
+    double *oAlpha = nullptr;   // Delta-CDF weight of every oscillator (kind_of_DR = 4 only)
 };
 
 struct McState;  // transport buffers, trk_mc.hip

@@ -207,6 +207,57 @@ static double int_ritchi_p_x(double A, double E, double G, double x)
     return In.real();
 }
 
+// Int_Ritchi_x as Reading_files_and_parameters.f90:2208-2239 has it, in
+// explicit complex arithmetic (naive products and quotients, principal
+// square root, log as log|z| + i arg z) so that it rounds the same on every
+// host: the k-sum antiderivative of one Ritchie-Drude oscillator
+namespace {
+struct Cx { double re, im; };
+Cx cx(double r, double i) { return Cx{r, i}; }
+Cx cadd(Cx a, Cx b) { return cx(a.re + b.re, a.im + b.im); }
+Cx csub(Cx a, Cx b) { return cx(a.re - b.re, a.im - b.im); }
+Cx cmul(Cx a, Cx b) { return cx(a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re); }
+Cx cdiv(Cx a, Cx b)
+{
+    const double d = b.re * b.re + b.im * b.im;
+    return cx((a.re * b.re + a.im * b.im) / d, (a.im * b.re - a.re * b.im) / d);
+}
+Cx cscale(Cx a, double s) { return cx(a.re * s, a.im * s); }
+Cx csqrt_p(Cx z)
+{
+    const double r = std::sqrt(z.re * z.re + z.im * z.im);
+    const double x = std::sqrt(0.5 * (r + std::fabs(z.re)));
+    if (x == 0.0) return cx(0.0, 0.0);
+    if (z.re >= 0.0) return cx(x, z.im / (2.0 * x));
+    return cx(std::fabs(z.im) / (2.0 * x), (z.im >= 0.0) ? x : -x);
+}
+Cx clog_p(Cx z) { return cx(std::log(std::hypot(z.re, z.im)), std::atan2(z.im, z.re)); }
+
+double int_ritchi_x_rf(double A, double E, double G, double x)
+{
+    const double sq2 = std::sqrt(2.0);
+    const Cx I = cx(0.0, 1.0), one = cx(1.0, 0.0);
+    const double g2 = G * G;
+    const double d = -(2.0 * E) * (2.0 * E) + g2;
+    const Cx Sc = (d >= 0.0) ? cx(std::sqrt(d), 0.0) : cx(0.0, std::sqrt(std::fabs(d)));
+    const Cx Gc = cx(g2 - (2.0 * E) * E, 0.0);
+    const Cx GammaSc = cscale(Sc, G);
+    const Cx s_plus = cscale(csqrt_p(cadd(Gc, GammaSc)), sq2);
+    const Cx s_minus = cscale(csqrt_p(csub(Gc, GammaSc)), sq2);
+    const Cx Bc = cdiv(Gc, GammaSc);
+    Cx Ic1 = cx(0.0, 0.0);
+    if (!(s_minus.re == 0.0 && s_minus.im == 0.0)) {
+        const Cx arg = cdiv(cx(2.0 * x, 0.0), s_minus);
+        const Cx L = csub(clog_p(csub(one, cmul(I, arg))), clog_p(cadd(one, cmul(I, arg))));
+        Ic1 = cmul(cdiv(cmul(cscale(I, 0.5), L), s_minus), csub(one, Bc));
+    }
+    const Cx arg2 = cdiv(cx(2.0 * x, 0.0), s_plus);
+    const Cx L2 = csub(clog_p(csub(one, cmul(I, arg2))), clog_p(cadd(one, cmul(I, arg2))));
+    const Cx Ic = cadd(Ic1, cmul(cdiv(cmul(cscale(I, 0.5), L2), s_plus), cadd(one, Bc)));
+    return cscale(Ic, A * G).re;
+}
+}  // namespace
+
 extern "C" {
 
 double trk_det_log(double x) { return trk_log(x); }
@@ -288,6 +339,15 @@ double trk_w_plasma(double density_m3, double mass_kg)
 {
     if (mass_kg > 0.0) return ((density_m3 * trk::kE) * trk::kE) / (trk::kE0 * mass_kg);
     return ((density_m3 * trk::kE) * trk::kE) / (trk::kE0 * trk::kMe);
+}
+
+// define_alpha, Reading_files_and_parameters.f90:2199-2206: the weight of an
+// oscillator's delta peak, its k-sum above x_min
+double trk_delta_alpha(double A, double E0, double Gamma, double x_min)
+{
+    const double low_lim = int_ritchi_x_rf(A, E0, Gamma, x_min);
+    const double high_lim = int_ritchi_x_rf(A, E0, Gamma, 1.0e30);
+    return high_lim - low_lim;
 }
 
 // barrier_parameters, Monte_Carlo.f90:2094-2116: the energy E1 at which the

@@ -5,6 +5,7 @@
 
 #include <hip/hip_runtime.h>
 #include "trk_ctx.h"
+#include "trk_delta.h"
 
 namespace trk {
 
@@ -137,6 +138,8 @@ struct McDev {
     const double *RE;
     int *em_n, *em_h;
     unsigned long long *em_e;
+    // the oscillators' Delta-CDF weights (kind_of_DR = 4), indexed like m.oE0
+    const double *delta_alpha;
 };
 
 enum { EV_SHI = 0, EV_E_INEL, EV_E_EL, EV_H_INEL, EV_H_EL, EV_AUGER, EV_RAD, EV_PHOT };
@@ -176,7 +179,7 @@ constexpr int kTransportBlockMx = 768;
 // FP64, replays the oracle bit for bit) and mx (mixed precision); these are
 // the kernels the host driver launches from each.
 #define TRK_MC_KERNELS                                                          \
-    template <bool kAll> __global__ void k_transport(McDev din);             \
+    template <bool kAll, bool kDelta> __global__ void k_transport(McDev din); \
     __global__ void k_ion_init(McDev d, int nb, uint32_t first_iter);        \
     __global__ void k_shi_consts(IonDev iv, const double *Ip, int nsh, int *sd_M, double *sd_c); \
     __global__ void k_shi_buckets(IonDev iv, int nsh, const int *sd_M, const double *sd_c, int *sd_bk); \

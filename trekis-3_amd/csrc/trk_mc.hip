@@ -462,6 +462,7 @@ void fill_mcdev(const trk_ctx *c, McDev &d)
     d.ln_rhw_el = ts.ln_hw[TRK_TAB_E_EL]; d.ln_rlam_el = ts.ln_lam[TRK_TAB_E_EL];
     d.ln_rhw_hel = ts.ln_hw[TRK_TAB_H_EL]; d.ln_rlam_hel = ts.ln_lam[TRK_TAB_H_EL];
     d.lnES = ts.ln_E[TRK_TAB_SHI]; d.ln_LSsh = ts.ln_L[TRK_TAB_SHI]; d.ln_shi_path = ts.ln_shi_path;
+    d.delta_alpha = (c->base ? c->base : c)->mb.oAlpha;
     d.em_W = d.em_H = d.em_E1 = d.em_gamma = 0.0;
     const trk_material &m = c->hm.m;
     if (m.work_function > 0.0 &&
@@ -641,6 +642,17 @@ struct NoAllocBufs {
 static int finalize_on_device(trk_ctx *c, McState *s, const NoAllocBufs &na, const double *Out_V, int NR,
                               trk_tallies *out, bool want_spec);
 
+// The transport kernel of a run: precision, all tables staged in LDS, the
+// Delta-CDF model's inelastic transfers
+static const void *transport_kernel(bool mixed, bool all, bool delta)
+{
+    if (delta)
+        return mixed ? (all ? (const void *)mx::k_transport<true, true> : (const void *)mx::k_transport<false, true>)
+                     : (all ? (const void *)fp64::k_transport<true, true> : (const void *)fp64::k_transport<false, true>);
+    return mixed ? (all ? (const void *)mx::k_transport<true, false> : (const void *)mx::k_transport<false, false>)
+                 : (all ? (const void *)fp64::k_transport<true, false> : (const void *)fp64::k_transport<false, false>);
+}
+
 // The run itself on buffers the caller of this function provides in s (and
 // na, for a run without allocation: the tallies are then finished on the
 // device instead of the host).
@@ -650,6 +662,7 @@ static int run_core(trk_ctx *c, McState *s, int64_t nmc, uint64_t seed, int64_t 
 {
     auto tstart = std::chrono::steady_clock::now();
     const bool mixed = c->hm.o.precision == 1;   // FP32 elementary functions in the event code
+    const bool delta = c->hm.o.kind_of_DR == 4;  // the Delta-CDF instantiation of the transport kernel
     const HostMat &h = c->hm;
     const int Nt = h.Nt, nsh = h.nsh;
     hipStream_t st = c->stream;
@@ -739,8 +752,7 @@ static int run_core(trk_ctx *c, McState *s, int64_t nmc, uint64_t seed, int64_t 
         }
         const bool all = plan_lds(d, Nt, NR, h.m.n_dos, nsh, limit, &s->lds_bytes);
         s->all_staged = all;
-        const void *kt = mixed ? (all ? (const void *)mx::k_transport<true> : (const void *)mx::k_transport<false>)
-                               : (all ? (const void *)fp64::k_transport<true> : (const void *)fp64::k_transport<false>);
+        const void *kt = transport_kernel(mixed, all, delta);
         if (s->lds_bytes > 65536)
             HIPC(hipFuncSetAttribute(kt, hipFuncAttributeMaxDynamicSharedMemorySize, (int)s->lds_bytes));
         int per_cu = 0;
@@ -834,10 +846,19 @@ static int run_core(trk_ctx *c, McState *s, int64_t nmc, uint64_t seed, int64_t 
             d.cursor = s->d_counts + 8;
             HIPC(hipMemsetAsync(d.cursor, 0, sizeof(int), st));
             if (c->prof) HIPC(hipEventRecord(c->pe[0], st));
-            if (mixed && s->all_staged) mx::k_transport<true><<<(unsigned)nblk, tbl, s->lds_bytes, st>>>(d);
-            else if (mixed) mx::k_transport<false><<<(unsigned)nblk, tbl, s->lds_bytes, st>>>(d);
-            else if (s->all_staged) fp64::k_transport<true><<<(unsigned)nblk, tbl, s->lds_bytes, st>>>(d);
-            else fp64::k_transport<false><<<(unsigned)nblk, tbl, s->lds_bytes, st>>>(d);
+            const unsigned g = (unsigned)nblk;
+            const size_t lds = s->lds_bytes;
+            if (delta) {
+                if (mixed && s->all_staged) mx::k_transport<true, true><<<g, tbl, lds, st>>>(d);
+                else if (mixed) mx::k_transport<false, true><<<g, tbl, lds, st>>>(d);
+                else if (s->all_staged) fp64::k_transport<true, true><<<g, tbl, lds, st>>>(d);
+                else fp64::k_transport<false, true><<<g, tbl, lds, st>>>(d);
+            } else {
+                if (mixed && s->all_staged) mx::k_transport<true, false><<<g, tbl, lds, st>>>(d);
+                else if (mixed) mx::k_transport<false, false><<<g, tbl, lds, st>>>(d);
+                else if (s->all_staged) fp64::k_transport<true, false><<<g, tbl, lds, st>>>(d);
+                else fp64::k_transport<false, false><<<g, tbl, lds, st>>>(d);
+            }
             HIPC(hipGetLastError());
             if (c->prof) { HIPC(hipEventRecord(c->pe[1], st)); pending = 1; }
             launches++;

@@ -7,8 +7,10 @@ namespace fp64 {
 #include "trk_events.inc"
 #undef TRK_MX
 
-template __global__ void k_transport<true>(McDev din);
-template __global__ void k_transport<false>(McDev din);
+template __global__ void k_transport<true, false>(McDev din);
+template __global__ void k_transport<false, false>(McDev din);
+template __global__ void k_transport<true, true>(McDev din);
+template __global__ void k_transport<false, true>(McDev din);
 template __global__ void k_batch<true>(BatchCtl *ctl, int n_groups);
 template __global__ void k_batch<false>(BatchCtl *ctl, int n_groups);
 }  // namespace fp64

@@ -99,9 +99,10 @@ struct TableSet {
 struct HostMat;  // defined in trk_ctx.h
 
 // mat: when given, the material kinds are taken from that (built) table set
-// and only the ion's tables are computed
+// and only the ion's tables are computed.  delta_alpha: the oscillators'
+// Delta-CDF weights (device memory) when kind_of_DR = 4.
 int build_tables(const HostMat &hm, const MatDev &md, TableSet &ts, hipStream_t st, std::string &err,
-                 const TableSet *mat = nullptr);
+                 const TableSet *mat = nullptr, const double *delta_alpha = nullptr);
 void free_tables(TableSet &ts);
 int refresh_derived(const HostMat &hm, TableSet &ts, hipStream_t st, std::string &err);
 

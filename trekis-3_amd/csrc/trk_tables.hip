@@ -33,6 +33,7 @@
 #include <string>
 #include <vector>
 #include "trk_ctx.h"
+#include "trk_delta.h"
 
 namespace trk {
 
@@ -640,6 +641,24 @@ __global__ void k_phot(MatDev m, const double *__restrict__ E, int n, double *__
 }
 
 // How_many_electrons, MC:1990-2055: total MFP = 1/sum(1/lambda_shell)
+// TotIMFP's Delta-CDF branch, CS:952-956: per (energy, shell) of an electron
+// or hole inelastic table, the mean free path and stopping power of the
+// delta peaks (with the electron's mass for holes too, as the reference),
+// over the quadrature's values
+__global__ void k_delta_tables(MatDev m, const double *__restrict__ alpha, const double *__restrict__ E, int n,
+                               int nsh, double *__restrict__ L, double *__restrict__ dEdx)
+{
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (int64_t)n * nsh) return;
+    const int s = (int)(t / n), i = (int)(t % n);
+    double Emin = m.Ip[s];
+    if (Emin <= 1.0e-3) Emin = 1.0e-3;
+    const int o0 = m.osc_off[s], o1 = m.osc_off[s + 1];
+    const double Sigma = delta_cs(m.oE0, alpha, o0, o1, E[i], Emin, m.At_Dens, 0.0);
+    L[(size_t)s * n + i] = delta_mfp(Sigma, m.At_Dens);
+    dEdx[(size_t)s * n + i] = delta_eloss(m.oE0, alpha, o0, o1, E[i], Emin, m.At_Dens);
+}
+
 __global__ void k_fuse(const double *__restrict__ L, int n, int nsh, int skip_small, double *__restrict__ out)
 {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -882,7 +901,7 @@ static void borrow_material(TableSet &ts, const TableSet &mat)
 }
 
 int build_tables(const HostMat &h, const MatDev &md, TableSet &ts, hipStream_t st, std::string &err,
-                 const TableSet *mat)
+                 const TableSet *mat, const double *delta_alpha)
 {
     auto t_start = std::chrono::steady_clock::now();
     free_tables(ts);
@@ -1044,6 +1063,14 @@ int build_tables(const HostMat &h, const MatDev &md, TableSet &ts, hipStream_t s
             k_inv_row<<<nblk(dn[s], 256), 256, 0, st>>>(td.dL + td.doff[s], dn[s], td.dInv + td.doff[s], td.dNonMono + s);
     HIP_OK(hipGetLastError());
 
+    // the Delta-CDF model's mean free paths and stopping powers
+    if (own && h.o.kind_of_DR == 4) {
+        if (!delta_alpha) { err = "the Delta-CDF model needs the oscillators' weights"; return TRK_ERR_STATE; }
+        for (int k : {TRK_TAB_E_INEL, TRK_TAB_H_INEL})
+            k_delta_tables<<<nblk((int64_t)T[k].n * nsh, 128), 128, 0, st>>>(md, delta_alpha, T[k].E, T[k].n, nsh,
+                                                                             T[k].L, T[k].dEdx);
+        HIP_OK(hipGetLastError());
+    }
     // Mott / disabled elastic tables
     if (own && h.o.kind_of_EMFP != 1) {
         k_mott<<<nblk(T[TRK_TAB_E_EL].n, 128), 128, 0, st>>>(md, T[TRK_TAB_E_EL].E, T[TRK_TAB_E_EL].n, 0,

@@ -162,6 +162,7 @@ def declare_library(lib):
         "trk_sumrules": (c_int, [DP, DP, DP, c_int32, c_double, c_double, DP, DP]),
         "trk_w_plasma": (c_double, [c_double, c_double]),
         "trk_barrier_parameters": (c_int, [c_double, c_double, c_double, DP, DP]),
+        "trk_delta_alpha": (c_double, [c_double, c_double, c_double, c_double]),
         "trk_dos_size": (c_int, [DP, c_int32, P(c_int32)]),
         "trk_prepare_dos": (c_int, [DP, DP, c_int32, c_double, c_double, c_double,
                                     DP, DP, DP, DP, DP, DP, DP, c_int32]),

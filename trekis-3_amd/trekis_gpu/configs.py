@@ -417,6 +417,10 @@ def config_material(name):
         ion.Kind_ion = rest[0] if rest else 0
         if len(rest) > 1:
             mat.work_function, mat.bar_length, mat.bar_height = rest[1]
+        if dr == 4:
+            # electrons slowed below the peaks scatter elastically for the
+            # rest of the window: ten times the events per iteration
+            run = dict(run, nmc=2)
         opt.kind_of_DR, opt.plasmon_Emax = dr, plasmon
         opt.dt_flag, opt.dt = dt_flag, dt
         return mat, ion, opt, run
@@ -443,6 +447,8 @@ def config_material(name):
 #   opt_bk_ion        a Brandt-Kitagawa ion (Kind_ion = 1, a 9th entry): its
 #                     mean free paths with the charge form factor inside the q
 #                     integral (SHI_TotIMFP_BK, CS:2748-2889), Bohr charge
+#   opt_delta_cdf     the Delta-CDF model of the electron and hole inelastic
+#                     cross sections (kind_of_DR = 4, CS:952-956, 1894-1896)
 #   opt_emission      emission through the surface Z = 0 (a 10th entry: work
 #                     function [eV], barrier length [A], barrier height [eV];
 #                     calculate_emission, MC:2477-2513): electrons above 15 eV
@@ -455,6 +461,7 @@ OPTION_WORKLOADS = {
     "opt_fixed_zeff": ("test_metal", 1, 4, 3.5, 300.0, 0, 1, 10.0),
     "opt_bk_ion": ("test", 1, 1, 8.0, 0.0, 0, 0, 1.0, 1),
     "opt_emission": ("test", 1, 0, 8.0, 0.0, 0, 0, 1.0, 0, (4.5, 10.0, 10.0)),
+    "opt_delta_cdf": ("test", 4, 0, 8.0, 0.0, 0, 0, 1.0),
 }
 
 
