@@ -96,6 +96,16 @@ typedef struct trk_material {
     const double *dos_Eff_m;            /* [me] */
     const double *dos_k_inv;
     const double *dos_Eff_m_inv;
+
+    /* Inelastic cross section of electrons and holes per shell (KOCS,
+     * Objects.f90:273): 0 or 1 the shell's CDF, 2 the binary-encounter-Bethe
+     * formula of Kim and Rudd with the shell's mean kinetic energy Ek [eV]
+     * (Sigma_BEB / dSigma_int_BEB, Cross_sections.f90:3888-4044; transfers by
+     * Electron_NRG_transfer_BEB, CS:2126-2161).  The ion always uses the CDF
+     * (the reference marks its BEB branch as not working, MC:1772), so every
+     * shell keeps at least one oscillator. */
+    int32_t kocs[TRK_MAX_SHELLS];
+    double  Ek[TRK_MAX_SHELLS];
 } trk_material;
 
 /* Swift heavy ion (Objects.f90:44-53). */

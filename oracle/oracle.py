@@ -65,6 +65,10 @@ def load():
         "or_delta_cs": (c_double, [c_void_p, c_int, c_double, c_double, c_int, c_double]),
         "or_delta_eloss": (c_double, [c_void_p, c_int, c_double, c_double]),
         "or_delta_transfer": (c_double, [c_void_p, c_int, c_double, c_double, c_double]),
+        "or_beb_sigma": (c_double, [c_double, c_double, c_double, c_double]),
+        "or_beb_dsigma_int": (c_double, [c_double, c_double, c_double, c_double, c_double]),
+        "or_beb_dsigma_w_int": (c_double, [c_double, c_double, c_double, c_double, c_double]),
+        "or_beb_transfer": (c_double, [c_void_p, c_int, c_double, c_double, c_double, c_double]),
         "or_sample_transfer": (c_double, [c_void_p, c_int, c_int, c_double, c_double]),
         "or_mott_theta": (c_double, [c_void_p, c_int, c_double, c_double, DP]),
         "or_mott_W": (c_double, [c_void_p, c_int, c_double, c_double, c_double]),

@@ -90,6 +90,12 @@ double or_delta_cs(const or_ctx *c, int shell, double E, double Ip, int has_emax
 double or_delta_eloss(const or_ctx *c, int shell, double E, double Ip);
 void   or_delta_tot_imfp(const or_ctx *c, int shell, double Ele, double Emin, double *L, double *dEdx);
 double or_delta_transfer(const or_ctx *c, int shell, double Ee, double Ip, double RN);
+/* the BEB cross section (trk_oracle_beb.c) */
+double or_beb_sigma(double T, double B, double U, double N);
+double or_beb_dsigma_int(double T, double w, double B, double U, double N);
+double or_beb_dsigma_w_int(double T, double w, double B, double U, double N);
+void   or_beb_tot_imfp(const or_ctx *c, int shell, double Ele, double Mass, double *L, double *dEdx);
+double or_beb_transfer(const or_ctx *c, int shell, double Ele, double L_need, double Mass, double Emin);
 double or_nfp_1d(or_ctx *c, double E, const double *Earr, const double *Larr, int32_t n);
 double or_nfp_2d(or_ctx *c, double E, const double *Earr, const double *Larr, int32_t n);
 double or_interp_transfer(or_ctx *c, const or_table *t, int shell, double Ele, double L_need);

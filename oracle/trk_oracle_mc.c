@@ -297,9 +297,38 @@ static double transfer_inel_delta(or_mc *S, double Ele, int shell, int hole, uin
     return E;
 }
 
+/* Electron_energy_transfer_inelastic on a BEB shell (KOCS = 2, CS:1841-1848):
+ * Electron_NRG_transfer_BEB's search for L_need */
+static double transfer_inel_beb(or_ctx *c, double Ele, int shell, double L_tot, int hole, double RN)
+{
+    double L_need = L_tot / RN;
+    double Emin = c->m.Ip[shell];
+    double Egap = c->Egap;
+    double Emax, Mass;
+    if (Emin <= 1.0e-3) Emin = 1.0e-3;
+    if (!hole) {
+        Emax = (Ele + Emin) / 2.0;
+        Mass = 1.0;
+    } else {
+        Mass = hole_mass_tab(c, Ele);
+        Emax = ((4.0 * Ele) * Mass) / ((Mass + 1.0) * (Mass + 1.0));
+    }
+    if (c->o.plasmon_Emax && Emin == Egap) {
+        double Ep = SQRT((((((c->m.N_VB_el * c->m.At_Dens) * 1.0e6) * OR_H) * OR_H) / (OR_ME * OR_E0)) + (Egap * Egap));
+        if (Ep >= Emax) Emax = Ep;
+        if (Ele < Emax) Emax = Ele;
+    }
+    double E = or_beb_transfer(c, shell, Ele, L_need, Mass, Emin);
+    if (E < Emin) E = Emin;
+    if (E > Emax) E = Emax;
+    if (E != E) E = Emin;
+    return E;
+}
+
 static double e_transfer_inel(or_mc *S, double Ele, int shell, double L_tot, int hole, uint64_t id, uint32_t *nd)
 {
     double RN = rn(S, id, nd);
+    if (S->c->m.kocs[shell] == 2) return transfer_inel_beb(S->c, Ele, shell, L_tot, hole, RN);
     if (S->c->o.kind_of_DR == 4) return transfer_inel_delta(S, Ele, shell, hole, id, nd);
     return transfer_inel_rn(S->c, Ele, shell, L_tot, hole, RN);
 }

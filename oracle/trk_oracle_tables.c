@@ -481,6 +481,8 @@ static void tot_imfp(const or_ctx *c, double Ele, int shell, int kind, double *S
     /* the Delta-CDF model's closed forms in place of the quadrature's totals
      * (CS:952-956); the rows stay the quadrature's */
     if (c->o.kind_of_DR == 4 && kind <= 1) or_delta_tot_imfp(c, shell, Ele, Emin, Sigma, dEdx);
+    /* a BEB shell's closed forms (TotIMFP's KOCS branch, CS:950, 1035-1041) */
+    if (c->m.kocs[shell] == 2 && kind <= 1) or_beb_tot_imfp(c, shell, Ele, Mass, Sigma, dEdx);
     if (row) {
         for (int k = 0; k < cap; ++k) {
             if (cum[k] < 1.0e-15) cum[k] = 1.0e15;

@@ -35,7 +35,8 @@ def _descriptor(m, ion, opt, nmc, seed, Nt, R, V, ms):
     for a in range(m.n_atoms):
         i4(m.Zat[a]); f8(m.Mass[a], m.Pers[a]); i4(m.n_shells[a])
         for _ in range(m.n_shells[a]):
-            f8(m.Ip[s], m.Nel[s], m.Auger[s], m.Radiat[s]); i4(m.Shl_num[s], m.n_osc[s])
+            f8(m.Ip[s], m.Nel[s], m.Auger[s], m.Radiat[s]); i4(m.Shl_num[s], m.n_osc[s], m.kocs[s] or 1)
+            f8(m.Ek[s])
             for _ in range(m.n_osc[s]):
                 f8(m.osc_E0[o], m.osc_A[o], m.osc_Gamma[o])
                 o += 1
@@ -58,7 +59,7 @@ def _descriptor(m, ion, opt, nmc, seed, Nt, R, V, ms):
 
 @pytest.mark.skipif(not os.path.exists(DRIVER), reason="the Fortran bridge driver is built where the reference's "
                                                        "Objects.f90 is (make -C trekis-3_amd/fortran)")
-@pytest.mark.parametrize("name", ["test", "test_metal", "opt_emission"])
+@pytest.mark.parametrize("name", ["test", "test_metal", "opt_emission", "opt_beb"])
 def test_fortran_do_monte_carlo_equals_c_abi(name, tmp_path):
     lib = tg.load_library()
     mat, ion, opt, run = tg.config_material(name)

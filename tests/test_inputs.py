@@ -69,11 +69,26 @@ def test_ionisation_potential_floor_and_no_phonon_block():
 @pytest.mark.parametrize("edit,msg", [
     (("2		! elements\n13	2	! Z, contribution\n8	3\n", "Al2O3\n"), "EADL"),
     (("2	2	80.0d0	8	27.27d0", "0	2	80.0d0	8	27.27d0"), "BEB"),
-    (("2	2	80.0d0	8	27.27d0", "2	-2	80.0d0	8	27.27d0"), "BEB"),
 ])
 def test_forms_needing_other_inputs_are_rejected(edit, msg):
     with pytest.raises(ValueError, match=msg):
         inputs.parse_cdf(TWO_ELEMENTS.replace(*edit))
+
+
+def test_negative_designator_is_a_beb_shell():
+    """A negative designator keeps the shell's oscillators for the ion and
+    gives its electrons and holes the BEB cross section (RF:1556-1561): the
+    shell becomes kocs = 2 with the given mean kinetic energy, and without
+    one it is refused (the reference reads it from EADL)."""
+    t = TWO_ELEMENTS.replace("2	2	80.0d0	8	27.27d0", "2	-2	80.0d0	8	27.27d0")
+    f = inputs.parse_cdf(t)
+    assert f.atoms[0]["shells"][0]["beb"] and f.atoms[0]["shells"][0]["shl_num"] == 2
+    E, D = configs.dos_insulator()
+    with pytest.raises(ValueError, match="EADL"):
+        inputs.read_cdf(t, layer=100.0, dos=(E, D))
+    mat = inputs.read_cdf(t, layer=100.0, dos=(E, D), ek=[[110.0, None], [None]])
+    sh = mat.atoms[0].shells[0]
+    assert (sh.kocs, sh.Ek) == (2, 110.0) and mat.atoms[0].shells[1].kocs == 1
 
 
 def test_records_span_blank_lines_and_bad_values_stop():

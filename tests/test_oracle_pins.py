@@ -40,6 +40,13 @@ the reference's expressions:
   linear extrapolation meets the model at its switching energy and vanishes
   at the edge; the sampled transfer's integrated cross section is the drawn
   fraction of the total.
+* the BEB cross section (KOCS = 2; Sigma_BEB, dSigma_int_BEB,
+  dSigma_w_int_BEB, CS:3888-4044; Electron_NRG_transfer_BEB CS:2126-2161)
+  against Kim and Rudd's singly differential BEB cross section (Phys. Rev.
+  A 50, 3954 (1994), Eq. (52) with Q = 1): the total is its integral over
+  [0, (T - B)/2], the cumulative form's derivative is it, the energy-weighted
+  form is the integral of W times it; a sampled transfer's cumulative mean
+  free path is the target one to the search's 1e-3.
 * barrier_parameters (MC:2094-2116): the logistic transmission
   1/(1 + exp(gamma (E1 - E))) of the surface is fitted to Eckart's
   transmission through a barrier of height H, work function W and length L
@@ -61,6 +68,7 @@ from oracle import Oracle
 HBARC = 1973.269804          # eV A
 MEC2 = 510998.95             # eV
 A0 = 0.529177210903          # A
+RY = 13.605693122994          # eV
 ALPHA = 7.2973525693e-3
 E2 = 14.399645               # e^2 / (4 pi eps0), eV A
 HBAR = 1.054571817e-34       # J s
@@ -450,4 +458,67 @@ def test_delta_cdf_linear_branch_and_sampling(lib):
                 ok = abs(got - RN * tot) <= 1e-3 * RN * tot
                 lo, hi = o.lib.or_delta_cs(o.h, s, E, Ip, 1, dE - 1e-3), o.lib.or_delta_cs(o.h, s, E, Ip, 1, dE + 1e-3)
                 assert ok or lo <= RN * tot <= hi, (s, E, RN, dE, got / tot)
+    o.close()
+
+
+def _beb_dsdw(W, T, B, U, N):
+    """Kim and Rudd's BEB d sigma / dW [A^2/eV] (Q = 1, N_i = N)."""
+    S = 4.0 * math.pi * A0 * A0 * N * (RY / B) ** 2
+    t, u, w = T / B, U / B, W / B
+    f = (-(1.0 / (w + 1.0) + 1.0 / (t - w)) / (t + 1.0) + (1.0 / (w + 1.0) ** 2 + 1.0 / (t - w) ** 2)
+         + math.log(t) * (1.0 / (w + 1.0) ** 3 + 1.0 / (t - w) ** 3))
+    return S / (B * (t + u + 1.0)) * f
+
+
+def test_beb_is_kim_rudd(lib):
+    import oracle as orc
+    olib = orc.load()
+    for T, B, U, N in ((200.0, 80.0, 105.0, 2.0), (1.0e3, 80.0, 105.0, 2.0), (5.0e3, 538.0, 700.0, 2.0),
+                       (1.0e5, 538.0, 700.0, 2.0), (40.0, 12.0, 30.0, 6.0)):
+        tot = olib.or_beb_sigma(T, B, U, N)
+        num = integrate.quad(_beb_dsdw, 0.0, (T - B) / 2.0, args=(T, B, U, N), epsrel=1e-13)[0]
+        # the reference's Rydberg and Bohr radius are 7e-7 from CODATA 2018's in S
+        assert tot == pytest.approx(num, rel=2e-6), (T, B)
+        for f in (0.05, 0.2, 0.45):
+            W = (T - B) * f
+            h = W * 1e-6
+            d = (olib.or_beb_dsigma_int(T, W + h, B, U, N) - olib.or_beb_dsigma_int(T, W - h, B, U, N)) / (2 * h)
+            assert d == pytest.approx(_beb_dsdw(W, T, B, U, N), rel=2e-6), (T, B, f)
+        Wt = (T - 1.0) / 2.0       # the upper limit TotIMFP gives the stopping power (CS:1039)
+        wint = integrate.quad(lambda W: W * _beb_dsdw(W, T, B, U, N), 0.0, Wt, epsrel=1e-13, limit=200)[0]
+        assert olib.or_beb_dsigma_w_int(T, Wt, B, U, N) == pytest.approx(wint, rel=2e-6), (T, B)
+    assert olib.or_beb_sigma(80.0, 80.0, 105.0, 2.0) == 0.0
+
+
+def test_beb_transfer_search(lib):
+    """The transfer of a BEB shell is B plus the W whose cumulative mean free
+    path 1/(n_a sigma(<W)) is the drawn L_need = L_total / RN: the search
+    (a bisection that stops one halving after its 1e-3 test passes) lands
+    within 3 % of L_need in the path or within 6 % of the exact root
+    (brentq) in the transfer -- the extra halving moves the one where the
+    path is steep and the other where it is flat -- and lower draws (longer
+    paths) give smaller transfers."""
+    from scipy import optimize
+    mat, ion, opt, run = tg.config_material("opt_beb")
+    mat.prepare(lib)
+    o = Oracle(mat, ion, opt)
+    shells = [sh for a in mat.atoms for sh in a.shells]
+    n = mat.At_Dens * 1e-24
+    for s in (0, 2):
+        sh = shells[s]
+        atom = [a for a in mat.atoms if sh in a.shells][0]
+        n_a = n * atom.pers / sum(a.pers for a in mat.atoms)
+        for T in (3.0 * sh.Ip, 10.0 * sh.Ip, 2.0e4):
+            L_tot = 1.0 / (n_a * o.lib.or_beb_sigma(T, sh.Ip, sh.Ek, sh.Nel))
+            prev = math.inf
+            for RN in (0.93, 0.5, 0.1):
+                L_need = L_tot / RN
+                E = o.lib.or_beb_transfer(o.h, s, T, L_need, 1.0, max(sh.Ip, 1e-3))
+                W = E - sh.Ip
+                assert 0.0 < W <= (T - sh.Ip) / 2.0
+                f = lambda w: 1.0 / (n_a * o.lib.or_beb_dsigma_int(T, w, sh.Ip, sh.Ek, sh.Nel)) - L_need
+                Wx = optimize.brentq(f, 1e-9 * T, (T - sh.Ip) / 2.0, xtol=1e-14, rtol=1e-14)
+                assert abs((f(W) + L_need) / L_need - 1.0) < 0.03 or abs(W / Wx - 1.0) < 0.06, (s, T, RN, W, Wx)
+                assert W < prev
+                prev = W
     o.close()

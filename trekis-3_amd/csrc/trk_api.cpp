@@ -49,6 +49,11 @@ static int validate(const trk_material *m, const trk_ion *ion, const trk_options
         if (!(m->Ip[s] > 0.0)) { why = "ionisation potentials must be positive"; return 1; }
     }
     if (nosc > TRK_MAX_OSC) { why = "too many oscillators"; return 1; }
+    for (int s = 0; s < nsh; ++s) {
+        if (m->kocs[s] < 0 || m->kocs[s] > 2) { why = "kocs must be 0 or 1 (CDF) or 2 (BEB)"; return 1; }
+        if (m->kocs[s] == 2 && !(m->Ek[s] >= 0.0 && m->Ek[s] < 1.0e30))
+            { why = "a BEB shell needs the mean kinetic energy Ek of its electrons (>= 0, finite)"; return 1; }
+    }
     if (m->n_phonon < 0 || m->n_phonon > TRK_MAX_PHONON) { why = "n_phonon out of range"; return 1; }
     if (o->kind_of_EMFP == 1 && m->n_phonon < 1) { why = "phonon-CDF elastic scattering needs phonon oscillators"; return 1; }
     if (o->kind_of_EMFP < -1 || o->kind_of_EMFP > 1) { why = "kind_of_EMFP must be -1, 0 or 1 (DSF is not built)"; return 1; }
@@ -148,6 +153,12 @@ int trk_ctx_create(const trk_material *mat, const trk_ion *ion, const trk_option
     rc |= upload(c, &b.oA, mat->osc_A, (size_t)o);
     rc |= upload(c, &b.oE0, mat->osc_E0, (size_t)o);
     rc |= upload(c, &b.oG, mat->osc_Gamma, (size_t)o);
+    {
+        std::vector<int> kocs((size_t)s);
+        for (int sh = 0; sh < s; ++sh) kocs[sh] = (mat->kocs[sh] == 2) ? 2 : 1;
+        rc |= upload(c, &b.kocs, kocs.data(), kocs.size());
+        rc |= upload(c, &b.Ek, mat->Ek, (size_t)s);
+    }
     if (opt->kind_of_DR == 4) {
         // the delta peaks' weights, each oscillator's k-sum above its
         // shell's edge (define_alpha, RF:1385-1388, 1572-1574)
@@ -219,7 +230,8 @@ void trk_ctx_destroy(trk_ctx *c)
     mc_free(c->mc);
     MatBuffers &b = c->mb;
     void *ps[] = {b.osc_off, b.atom_of, b.shl_off, b.Zat, b.oA, b.oE0, b.oG, b.phA, b.phE0, b.phG, b.k, b.Effm,
-                  b.dosE, b.dos, b.dosI, b.dosM, b.Ip, b.Nel, b.Auger, b.Radiat, b.Mass, b.Pers, b.mott, b.oAlpha};
+                  b.dosE, b.dos, b.dosI, b.dosM, b.Ip, b.Nel, b.Auger, b.Radiat, b.Mass, b.Pers, b.mott, b.oAlpha,
+                  b.kocs, b.Ek};
     for (void *p : ps) if (p) (void)hipFree(p);
     for (hipEvent_t e : c->pe) if (e) (void)hipEventDestroy(e);
     if (c->stream) (void)hipStreamDestroy(c->stream);

@@ -519,9 +519,9 @@ int trk_batch_create(trk_ctx *const *impacts, int32_t n, const double *Out_R, co
         if (c->tables_dirty || (c->base && c->base->tables_dirty))
             return berr(b, TRK_ERR_STATE, "impact %d: tables were set, trk_tables_finish must run first", i);
         if (c->hm.o.precision != impacts[0]->hm.o.precision) return berr(b, TRK_ERR_ARG, "impacts mix precisions");
-        if (c->hm.o.kind_of_DR == 4)
-            return berr(b, TRK_ERR_ARG, "impact %d uses the Delta-CDF model (kind_of_DR = 4), which trk_run_mc "
-                                        "transports and batches do not", i);
+        if (uses_general_kernel(c->hm))
+            return berr(b, TRK_ERR_ARG, "impact %d uses the Delta-CDF model (kind_of_DR = 4) or BEB shells, which "
+                                        "trk_run_mc transports and batches do not", i);
         if (c->device != impacts[0]->device) return berr(b, TRK_ERR_ARG, "impacts on different devices");
         const trk_ctx *m = c->base ? c->base : c;
         auto it = gi.find(m);

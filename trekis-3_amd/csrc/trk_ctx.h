@@ -36,6 +36,8 @@ struct MatBuffers {
     double *Mass = nullptr, *Pers = nullptr;
     double *mott = nullptr;     // [4][n_atoms]: Z, Z^(2/3), rest energy, Pers
     double *oAlpha = nullptr;   // Delta-CDF weight of every oscillator (kind_of_DR = 4 only)
+    int *kocs = nullptr;        // per shell: 1 CDF, 2 BEB (electrons and holes)
+    double *Ek = nullptr;       // per shell: mean kinetic energy of its electrons [eV] (BEB)
 };
 
 struct McState;  // transport buffers, trk_mc.hip

@@ -6,6 +6,7 @@
 #include <hip/hip_runtime.h>
 #include "trk_ctx.h"
 #include "trk_delta.h"
+#include "trk_beb.h"
 
 namespace trk {
 
@@ -138,8 +139,13 @@ struct McDev {
     const double *RE;
     int *em_n, *em_h;
     unsigned long long *em_e;
-    // the oscillators' Delta-CDF weights (kind_of_DR = 4), indexed like m.oE0
+    // the oscillators' Delta-CDF weights (kind_of_DR = 4), indexed like m.oE0;
+    // per shell, the electron and hole cross section (2: BEB) and the mean
+    // kinetic energy of the shell's electrons (both read by the general
+    // transport kernel only)
     const double *delta_alpha;
+    const int *beb_kocs;
+    const double *beb_Ek;
 };
 
 enum { EV_SHI = 0, EV_E_INEL, EV_E_EL, EV_H_INEL, EV_H_EL, EV_AUGER, EV_RAD, EV_PHOT };
@@ -179,7 +185,7 @@ constexpr int kTransportBlockMx = 768;
 // FP64, replays the oracle bit for bit) and mx (mixed precision); these are
 // the kernels the host driver launches from each.
 #define TRK_MC_KERNELS                                                          \
-    template <bool kAll, bool kDelta> __global__ void k_transport(McDev din); \
+    template <bool kAll, bool kGeneral> __global__ void k_transport(McDev din); \
     __global__ void k_ion_init(McDev d, int nb, uint32_t first_iter);        \
     __global__ void k_shi_consts(IonDev iv, const double *Ip, int nsh, int *sd_M, double *sd_c); \
     __global__ void k_shi_buckets(IonDev iv, int nsh, const int *sd_M, const double *sd_c, int *sd_bk); \
@@ -197,6 +203,8 @@ hipError_t launch_shi_consts(const trk_ctx *c, const IonDev &iv, bool mixed, hip
 void fill_mcdev(const trk_ctx *c, McDev &d);
 // LDS layout of the staged tables (trk_mc.hip); true when all are staged.
 bool plan_lds(McDev &d, int Nt, int NR, int n_dos, int nsh, size_t limit_doubles, size_t *bytes);
+// BEB shells or the Delta-CDF model: the transport kernel's general instantiation
+bool uses_general_kernel(const HostMat &h);
 // One tally plane converted and added into the caller's Out_* arrays (trk_mc.hip).
 int finish_tallies_host(trk_ctx *c, const Tally &T, int plane_index, const double *Out_V, int NR, trk_tallies *out,
                         hipStream_t st);

@@ -462,7 +462,10 @@ void fill_mcdev(const trk_ctx *c, McDev &d)
     d.ln_rhw_el = ts.ln_hw[TRK_TAB_E_EL]; d.ln_rlam_el = ts.ln_lam[TRK_TAB_E_EL];
     d.ln_rhw_hel = ts.ln_hw[TRK_TAB_H_EL]; d.ln_rlam_hel = ts.ln_lam[TRK_TAB_H_EL];
     d.lnES = ts.ln_E[TRK_TAB_SHI]; d.ln_LSsh = ts.ln_L[TRK_TAB_SHI]; d.ln_shi_path = ts.ln_shi_path;
-    d.delta_alpha = (c->base ? c->base : c)->mb.oAlpha;
+    const MatBuffers &mb = (c->base ? c->base : c)->mb;
+    d.delta_alpha = mb.oAlpha;
+    d.beb_kocs = mb.kocs;
+    d.beb_Ek = mb.Ek;
     d.em_W = d.em_H = d.em_E1 = d.em_gamma = 0.0;
     const trk_material &m = c->hm.m;
     if (m.work_function > 0.0 &&
@@ -642,11 +645,21 @@ struct NoAllocBufs {
 static int finalize_on_device(trk_ctx *c, McState *s, const NoAllocBufs &na, const double *Out_V, int NR,
                               trk_tallies *out, bool want_spec);
 
-// The transport kernel of a run: precision, all tables staged in LDS, the
-// Delta-CDF model's inelastic transfers
-static const void *transport_kernel(bool mixed, bool all, bool delta)
+// the transport kernel instantiation with the general inelastic transfers
+// runs when a shell uses the BEB cross section or the Delta-CDF model is on
+bool uses_general_kernel(const HostMat &h)
 {
-    if (delta)
+    if (h.o.kind_of_DR == 4) return true;
+    for (int s = 0; s < h.nsh; ++s)
+        if (h.m.kocs[s] == 2) return true;
+    return false;
+}
+
+// The transport kernel of a run: precision, all tables staged in LDS, the
+// general inelastic transfers (BEB shells, the Delta-CDF model)
+static const void *transport_kernel(bool mixed, bool all, bool general)
+{
+    if (general)
         return mixed ? (all ? (const void *)mx::k_transport<true, true> : (const void *)mx::k_transport<false, true>)
                      : (all ? (const void *)fp64::k_transport<true, true> : (const void *)fp64::k_transport<false, true>);
     return mixed ? (all ? (const void *)mx::k_transport<true, false> : (const void *)mx::k_transport<false, false>)
@@ -662,7 +675,7 @@ static int run_core(trk_ctx *c, McState *s, int64_t nmc, uint64_t seed, int64_t 
 {
     auto tstart = std::chrono::steady_clock::now();
     const bool mixed = c->hm.o.precision == 1;   // FP32 elementary functions in the event code
-    const bool delta = c->hm.o.kind_of_DR == 4;  // the Delta-CDF instantiation of the transport kernel
+    const bool general = uses_general_kernel(c->hm);   // BEB shells or the Delta-CDF model
     const HostMat &h = c->hm;
     const int Nt = h.Nt, nsh = h.nsh;
     hipStream_t st = c->stream;
@@ -752,7 +765,7 @@ static int run_core(trk_ctx *c, McState *s, int64_t nmc, uint64_t seed, int64_t 
         }
         const bool all = plan_lds(d, Nt, NR, h.m.n_dos, nsh, limit, &s->lds_bytes);
         s->all_staged = all;
-        const void *kt = transport_kernel(mixed, all, delta);
+        const void *kt = transport_kernel(mixed, all, general);
         if (s->lds_bytes > 65536)
             HIPC(hipFuncSetAttribute(kt, hipFuncAttributeMaxDynamicSharedMemorySize, (int)s->lds_bytes));
         int per_cu = 0;
@@ -848,7 +861,7 @@ static int run_core(trk_ctx *c, McState *s, int64_t nmc, uint64_t seed, int64_t 
             if (c->prof) HIPC(hipEventRecord(c->pe[0], st));
             const unsigned g = (unsigned)nblk;
             const size_t lds = s->lds_bytes;
-            if (delta) {
+            if (general) {
                 if (mixed && s->all_staged) mx::k_transport<true, true><<<g, tbl, lds, st>>>(d);
                 else if (mixed) mx::k_transport<false, true><<<g, tbl, lds, st>>>(d);
                 else if (s->all_staged) fp64::k_transport<true, true><<<g, tbl, lds, st>>>(d);

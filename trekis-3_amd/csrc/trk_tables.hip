@@ -34,6 +34,7 @@
 #include <vector>
 #include "trk_ctx.h"
 #include "trk_delta.h"
+#include "trk_beb.h"
 
 namespace trk {
 
@@ -1070,6 +1071,34 @@ int build_tables(const HostMat &h, const MatDev &md, TableSet &ts, hipStream_t s
             k_delta_tables<<<nblk((int64_t)T[k].n * nsh, 128), 128, 0, st>>>(md, delta_alpha, T[k].E, T[k].n, nsh,
                                                                              T[k].L, T[k].dEdx);
         HIP_OK(hipGetLastError());
+    }
+    // BEB shells (TotIMFP's BEB branch, CS:1035-1041): the closed forms of
+    // the mean free path and stopping power, on the host (a few values per
+    // shell), over the quadrature's values; the rows are not used
+    std::vector<std::vector<double>> beb_vals;
+    if (own) {
+        for (int k : {TRK_TAB_E_INEL, TRK_TAB_H_INEL}) {
+            const std::vector<double> &g = (k == TRK_TAB_E_INEL) ? gE : gH;
+            for (int s = 0; s < nsh; ++s) {
+                if (h.m.kocs[s] != 2) continue;
+                const double B = h.m.Ip[s], U = h.m.Ek[s], N = h.m.Nel[s];
+                const double temp1 = ((h.m.At_Dens * 1.0e-24) * h.m.Pers[h.atom_of[s]]) / h.sumPers;
+                std::vector<double> L(g.size()), D(g.size());
+                for (size_t i = 0; i < g.size(); ++i) {
+                    const double Ele = g[i];
+                    const double Mass = (k == TRK_TAB_H_INEL) ? hole_mass_at(h, Ele) : 1.0;
+                    L[i] = 1.0 / ((Mass * temp1) * beb_sigma(Ele, B, U, N));
+                    D[i] = (Mass * temp1) * beb_dsigma_w_int(Ele, (Ele - 1.0) / 2.0, B, U, N);
+                }
+                beb_vals.push_back(std::move(L));
+                HIP_OK(hipMemcpyAsync(T[k].L + (size_t)s * T[k].n, beb_vals.back().data(), sizeof(double) * g.size(),
+                                      hipMemcpyHostToDevice, st));
+                beb_vals.push_back(std::move(D));
+                HIP_OK(hipMemcpyAsync(T[k].dEdx + (size_t)s * T[k].n, beb_vals.back().data(), sizeof(double) * g.size(),
+                                      hipMemcpyHostToDevice, st));
+            }
+        }
+        if (!beb_vals.empty()) HIP_OK(hipStreamSynchronize(st));   // the host values are then free to go
     }
     // Mott / disabled elastic tables
     if (own && h.o.kind_of_EMFP != 1) {

@@ -53,11 +53,12 @@ program bridge_driver
       Target_atoms(a)%N_shl = nsh
       allocate(Target_atoms(a)%Ip(nsh), Target_atoms(a)%Nel(nsh), Target_atoms(a)%Auger(nsh))
       allocate(Target_atoms(a)%Radiat(nsh), Target_atoms(a)%Shl_num(nsh), Target_atoms(a)%Ritchi(nsh))
-      allocate(Target_atoms(a)%KOCS(nsh))
+      allocate(Target_atoms(a)%KOCS(nsh), Target_atoms(a)%Ek(nsh))
       Target_atoms(a)%KOCS = 1
       do k = 1, nsh
          read(u) Target_atoms(a)%Ip(k), Target_atoms(a)%Nel(k), Target_atoms(a)%Auger(k), Target_atoms(a)%Radiat(k)
-         read(u) i4, nosc
+         read(u) i4, nosc, Target_atoms(a)%KOCS(k)
+         read(u) Target_atoms(a)%Ek(k)
          Target_atoms(a)%Shl_num(k) = i4
          allocate(Target_atoms(a)%Ritchi(k)%E0(nosc), Target_atoms(a)%Ritchi(k)%A(nosc))
          allocate(Target_atoms(a)%Ritchi(k)%Gamma(nosc))

@@ -86,9 +86,22 @@ contains
                why = 'too many shells'
                return
             endif
+            m%kocs(s) = 1
+            m%Ek(s) = 0.0d0
+            ! the reference computes BEB for every KOCS other than 1 (CS:950)
             if (allocated(Target_atoms(a)%KOCS)) then
-               if (Target_atoms(a)%KOCS(k) /= 1) then
-                  why = 'BEB cross sections (KOCS /= 1) are not computed by the library'
+               if (Target_atoms(a)%KOCS(k) /= 1) m%kocs(s) = 2
+            endif
+            if (m%kocs(s) == 2) then
+               if (.not. allocated(Target_atoms(a)%Ek)) then
+                  why = 'BEB shells need the mean kinetic energies Ek of the shells'
+                  return
+               endif
+               m%Ek(s) = Target_atoms(a)%Ek(k)
+            endif
+            if (allocated(Target_atoms(a)%KOCS_SHI)) then
+               if (Target_atoms(a)%KOCS_SHI(k) /= 1) then
+                  why = 'the ion''s BEB cross section (KOCS_SHI /= 1) is not computed by the library'
                   return
                endif
             endif

@@ -34,6 +34,8 @@ module trekis_gpu_c
       real(c_double)     :: E_F, v_f, cut_off, temp, work_function, bar_length, bar_height
       integer(c_int32_t) :: n_dos
       type(c_ptr)        :: dos_E, dos_DOS, dos_int_DOS, dos_k, dos_Eff_m, dos_k_inv, dos_Eff_m_inv
+      integer(c_int32_t) :: kocs(TRK_MAX_SHELLS)
+      real(c_double)     :: Ek(TRK_MAX_SHELLS)
    end type trk_material
 
    type, bind(C) :: trk_ion

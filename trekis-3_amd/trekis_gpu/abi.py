@@ -67,6 +67,8 @@ class Material(ctypes.Structure):
         ("dos_Eff_m", DP),
         ("dos_k_inv", DP),
         ("dos_Eff_m_inv", DP),
+        ("kocs", c_int32 * MAX_SHELLS),
+        ("Ek", c_double * MAX_SHELLS),
     ]
 
 

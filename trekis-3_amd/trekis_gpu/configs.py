@@ -40,6 +40,8 @@ class Shell:
     radiat: float = 1.0e23                      # [fs]
     shl_num: int = 1
     normalize: bool = True                      # set A by the f-sum rule to Nel
+    kocs: int = 1                               # electron/hole cross section: 1 CDF, 2 BEB (KOCS)
+    Ek: float = 0.0                             # [eV] mean kinetic energy of the shell's electrons (BEB)
 
 
 @dataclass
@@ -170,6 +172,8 @@ class MaterialDesc:
                 m.Auger[s] = sh.auger
                 m.Radiat[s] = sh.radiat
                 m.Shl_num[s] = 63 if (ia == 0 and js == len(a.shells) - 1) else sh.shl_num
+                m.kocs[s] = sh.kocs
+                m.Ek[s] = sh.Ek
                 m.n_osc[s] = len(sh.osc)
                 for (E0, A, G) in sh.osc:
                     m.osc_E0[o], m.osc_A[o], m.osc_Gamma[o] = E0, A, G
@@ -415,8 +419,12 @@ def config_material(name):
         mat.temp = temp
         ion.Kind_Zeff, ion.fixed_Zeff = kz, fz
         ion.Kind_ion = rest[0] if rest else 0
-        if len(rest) > 1:
+        if len(rest) > 1 and rest[1]:
             mat.work_function, mat.bar_length, mat.bar_height = rest[1]
+        if len(rest) > 2:
+            shells = [sh for a in mat.atoms for sh in a.shells]
+            for s, Ek in rest[2].items():
+                shells[s].kocs, shells[s].Ek = 2, Ek
         if dr == 4:
             # electrons slowed below the peaks scatter elastically for the
             # rest of the window: ten times the events per iteration
@@ -447,6 +455,10 @@ def config_material(name):
 #   opt_bk_ion        a Brandt-Kitagawa ion (Kind_ion = 1, a 9th entry): its
 #                     mean free paths with the charge form factor inside the q
 #                     integral (SHI_TotIMFP_BK, CS:2748-2889), Bohr charge
+#   opt_beb           the BEB cross section (KOCS = 2) for the electrons and
+#                     holes on the two core shells (an 11th entry: flat shell
+#                     index -> mean kinetic energy of its electrons [eV]); the
+#                     ion keeps their CDF
 #   opt_delta_cdf     the Delta-CDF model of the electron and hole inelastic
 #                     cross sections (kind_of_DR = 4, CS:952-956, 1894-1896)
 #   opt_emission      emission through the surface Z = 0 (a 10th entry: work
@@ -462,6 +474,7 @@ OPTION_WORKLOADS = {
     "opt_bk_ion": ("test", 1, 1, 8.0, 0.0, 0, 0, 1.0, 1),
     "opt_emission": ("test", 1, 0, 8.0, 0.0, 0, 0, 1.0, 0, (4.5, 10.0, 10.0)),
     "opt_delta_cdf": ("test", 4, 0, 8.0, 0.0, 0, 0, 1.0),
+    "opt_beb": ("test", 1, 0, 8.0, 0.0, 0, 0, 1.0, 0, None, {0: 105.0, 2: 700.0}),
 }
 
 

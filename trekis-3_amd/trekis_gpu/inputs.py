@@ -7,12 +7,15 @@ configs.py build, so a material the reference reads from its input folder
 runs here unchanged.
 
 Supported is the file form the particle-history path needs: every shell
-with its own CDF oscillators (kind_of_CDF = 0).  Forms that need the EADL
-database or cross sections this library does not compute are rejected with
-the reason: a chemical formula in place of the element list
-(Decompose_compound), the single-pole file with VALENCE / PHONON markers
-(check_atomic_parameters for the shells), shells without oscillators or with
-a negative designator (BEB electron cross sections).
+with its own CDF oscillators (kind_of_CDF = 0); a negative designator gives
+the shell's electrons and holes the BEB cross section (KOCS = 2, RF:1556-1561),
+whose mean kinetic energy of the shell's electrons the caller passes (the
+reference reads it from EADL).  Forms that need the EADL database or cross
+sections this library does not compute are rejected with the reason: a
+chemical formula in place of the element list (Decompose_compound), the
+single-pole file with VALENCE / PHONON markers (check_atomic_parameters for
+the shells), shells without oscillators (the ion's BEB cross section, which
+the reference marks as not working, MC:1772).
 """
 import math
 import re
@@ -168,12 +171,13 @@ def _parse_cdf(text: str) -> CDFFile:
         (nshl,) = read("i")
         for k in range(nshl):
             ncdf, shl_num, Ip, Nel, auger = read("iirrr")
-            if ncdf <= 0 or shl_num < 0:
-                raise ValueError(f"element {at['Z']} shell {k + 1}: BEB electron cross sections "
-                                 "(no oscillators or a negative designator) are not computed by this library")
+            if ncdf <= 0:
+                raise ValueError(f"element {at['Z']} shell {k + 1}: a shell without oscillators takes the BEB "
+                                 "cross section for the ion too, which the reference marks as not working "
+                                 "(MC:1772) and this library does not compute")
             osc = [tuple(read("rrr")) for _ in range(ncdf)]     # E0, A, Gamma
             at["shells"].append(dict(ncdf=ncdf, shl_num=abs(shl_num), Ip=max(Ip, 0.1), Nel=Nel, auger=auger,
-                                     osc=osc))
+                                     osc=osc, beb=shl_num < 0))
     # Auger times as check_atomic_parameters takes them (Dealing_with_EADL.f90
     # :366-370): none for the valence band (designator >= 63, or the last shell
     # of the first element, where this library keeps the band); a time <= 0 or
@@ -219,21 +223,31 @@ def _text(path_or_text: str) -> str:
 
 
 def read_cdf(path_or_text: str, layer: float, dos: Tuple[np.ndarray, np.ndarray], *, cut_off: float = 10.0,
-             hole_mass: float = 1.0, el_eff_mass: float = 1.0, radiat=None) -> MaterialDesc:
+             hole_mass: float = 1.0, el_eff_mass: float = 1.0, radiat=None, ek=None) -> MaterialDesc:
     """MaterialDesc of a full-CDF material file.  The oscillators are used as
     the file gives them (no sum-rule renormalisation in this form); the last
     shell of the first element is the valence band.  radiat: radiative decay
     times per (element, shell) [fs] (the reference takes them from EADL;
-    default 1e23, no radiative decay).  layer, dos, cut_off, hole_mass and
-    el_eff_mass are the other input files' settings (INPUT_PARAMETERS.txt)."""
+    default 1e23, no radiative decay).  ek: mean kinetic energies of the
+    shells' electrons per (element, shell) [eV], needed for the shells with a
+    negative designator (BEB, from EADL in the reference).  layer, dos,
+    cut_off, hole_mass and el_eff_mass are the other input files' settings
+    (INPUT_PARAMETERS.txt)."""
     f = parse_cdf(_text(path_or_text))
     atoms = []
     for j, at in enumerate(f.atoms):
         shells = []
         for k, s in enumerate(at["shells"]):
             rad = 1.0e23 if radiat is None else radiat[j][k]
-            shells.append(Shell(s["Ip"], s["Nel"], list(s["osc"]), auger=s["auger"], radiat=rad,
-                                shl_num=s["shl_num"], normalize=False))
+            sh = Shell(s["Ip"], s["Nel"], list(s["osc"]), auger=s["auger"], radiat=rad, shl_num=s["shl_num"],
+                       normalize=False)
+            if s.get("beb"):
+                if ek is None or ek[j][k] is None:
+                    raise ValueError(f"element {at['Z']} shell {k + 1}: its BEB cross section needs the mean kinetic "
+                                     "energy of the shell's electrons, which the reference reads from the EADL "
+                                     "database; pass it as ek")
+                sh.kocs, sh.Ek = 2, float(ek[j][k])
+            shells.append(sh)
         atoms.append(Atom(at["Z"], ELEMENT_MASS[at["Z"]], at["pers"], shells))
     E, D = dos
     mat = MaterialDesc(f.name, atoms, dens=f.dens, layer=layer, vsound=f.vsound, E_F=f.E_F,
