@@ -121,7 +121,8 @@ typedef struct trk_ion {
 
 /* Numerical options (the Flag type, Objects.f90:135-180, and run controls). */
 typedef struct trk_options {
-    int32_t kind_of_EMFP;               /* -1 no elastic, 0 Mott, 1 phonon CDF */
+    int32_t kind_of_EMFP;               /* -1 no elastic, 0 Mott, 1 phonon CDF, 2 DSF (tables from
+                                           trk_dsf_set before trk_build_tables) */
     int32_t CDF_elast_Zeff;             /* 0 Barkas-like target charge, 1 unit charge */
     int32_t kind_of_DR;                 /* 1 free electron, 2 plasmon pole, 3 Ritchie, 4 Delta-CDF (the
                                            electron and hole inelastic cross sections of delta peaks,
@@ -229,6 +230,20 @@ int  trk_table_set(trk_ctx *ctx, int kind, int shell, const double *E, const dou
                    int32_t n);
 int  trk_row_set(trk_ctx *ctx, int kind, int shell, int32_t i_E, const double *hw, const double *lam, int32_t n);
 int  trk_tables_finish(trk_ctx *ctx);
+
+/* Elastic scattering from dynamic structure factors (kind_of_EMFP = 2):
+ * the differential mean free paths DSF_DEMFP (electrons, particle 0) or
+ * DSF_DEMFP_H (holes, particle 1) as the reference's reader leaves them
+ * (reading_DSF_cross_sections, Reading_files_and_parameters.f90:2516-2680):
+ * nE particle energies E (increasing) with nW transferred energies each,
+ * E-major arrays of the transferred energy dE, the cumulative mean free
+ * path dL and its emission (dE >= 0) and absorption parts dL_emit,
+ * dL_absorb [A].  The elastic mean free paths are their last entries
+ * (Analytical_IMFPs.f90:905-911); transfers are sampled as
+ * NRG_transfer_elastic_DSF does (Cross_sections.f90:3652-3795).  Call for
+ * both particles after trk_ctx_create and before trk_build_tables. */
+int  trk_dsf_set(trk_ctx *ctx, int32_t particle, int32_t nE, int32_t nW, const double *E, const double *dE,
+                 const double *dL, const double *dL_emit, const double *dL_absorb);
 
 /* Number of snapshot time nodes of the run (time_do loop, Monte_Carlo.f90:572-665). */
 int  trk_time_nodes(const trk_ctx *ctx, int32_t *Nt_out);

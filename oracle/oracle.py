@@ -61,6 +61,8 @@ def load():
         "or_time_nodes": (c_int, [c_void_p, P(c_int32), DP, c_int32]),
         "or_constant": (c_double, [c_int]),
         "or_barrier_parameters": (c_int, [c_double, c_double, c_double, DP, DP]),
+        "or_dsf_set": (c_int, [c_void_p, c_int, c_int32, c_int32, DP, DP, DP, DP, DP]),
+        "or_dsf_transfer": (c_double, [c_void_p, c_int, c_double, c_double, c_double]),
         "or_delta_alpha": (c_double, [c_double, c_double, c_double, c_double]),
         "or_delta_cs": (c_double, [c_void_p, c_int, c_double, c_double, c_int, c_double]),
         "or_delta_eloss": (c_double, [c_void_p, c_int, c_double, c_double]),
@@ -105,6 +107,10 @@ class Oracle:
         self.ion = ion
         self.opt = opt
         self.h = self.lib.or_create(ctypes.byref(self.mat), ctypes.byref(ion), ctypes.byref(opt))
+        if opt.kind_of_EMFP == 2 and getattr(material_desc, "dsf", None) is not None:
+            for particle, t in enumerate(material_desc.dsf):
+                E, dE, dL, Le, La = t.arrays()
+                self.lib.or_dsf_set(self.h, particle, dE.shape[0], dE.shape[1], dp(E), dp(dE), dp(dL), dp(Le), dp(La))
         nt = c_int32()
         self.lib.or_time_nodes(self.h, ctypes.byref(nt), None, 0)
         self.Nt = nt.value

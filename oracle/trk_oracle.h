@@ -87,6 +87,11 @@ double or_gap_walk(const or_ctx *c, double Ehole, int down);
 double or_constant(int which);
 /* barrier_parameters, MC:2094-2116; returns 0 when E1 and gamma are finite */
 int    or_barrier_parameters(double W, double bar_length, double H, double *E1, double *gamma);
+/* DSF elastic tables of electrons (particle 0) or holes (1), set before
+ * or_build_tables, and the transfer sampler for two given draws */
+int    or_dsf_set(or_ctx *c, int particle, int32_t nE, int32_t nW, const double *E, const double *dE,
+                  const double *dL, const double *dL_emit, const double *dL_absorb);
+double or_dsf_transfer(const or_ctx *c, int particle, double Eel, double RN1, double RN2);
 
 /* Monte Carlo (do_Monte_Carlo in the reference's global time order).  If
  * trace != NULL each processed event appends 12 doubles, the record layout of

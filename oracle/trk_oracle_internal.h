@@ -73,6 +73,9 @@ struct or_ctx {
     double em_W, em_H, em_E1, em_gamma;
     /* Delta-CDF weight of every oscillator (kind_of_DR = 4) */
     double alpha[TRK_MAX_OSC];
+    /* DSF tables of electrons [0] and holes [1] (kind_of_EMFP = 2) */
+    int32_t dsf_nE[2], dsf_nW[2];
+    double *dsfE[2], *dsfdE[2], *dsfL[2], *dsfLe[2], *dsfLa[2];
     /* time grid */
     int32_t Nt;
     double *tim_glob;
@@ -84,6 +87,9 @@ struct or_ctx {
 double or_imewq(const or_ctx *c, double hw, double hq, int shell, int photon);
 double or_shi_zeff(const or_ctx *c, double E);
 double or_rest_energy(double M0);
+/* DSF elastic scattering (trk_oracle_dsf.c) */
+void or_dsf_free(or_ctx *c);
+void or_dsf_elastic_table(const or_ctx *c, int particle, double **E, double **L, double **dEdx, int32_t *n);
 /* the Delta-CDF model (trk_oracle_delta.c) */
 double or_delta_alpha(double A, double E0, double Gamma, double x_min);
 double or_delta_cs(const or_ctx *c, int shell, double E, double Ip, int has_emax, double Emax_in);

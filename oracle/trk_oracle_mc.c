@@ -882,7 +882,12 @@ static void electron_event(or_mc *S, int idx)
     } else {
         kind = 2;
         double EMFPt = or_nfp_1d(c, Eel, c->tab[TRK_TAB_E_EL].E, c->tab[TRK_TAB_E_EL].L[0], c->tab[TRK_TAB_E_EL].n);
-        if (c->o.kind_of_EMFP == 1) {
+        if (c->o.kind_of_EMFP == 2) {
+            /* NRG_transfer_elastic_DSF, MC:2387-2390: two draws inside */
+            double r1 = rn(S, id, &nd);
+            double r2 = rn(S, id, &nd);
+            dE = or_dsf_transfer(c, 0, Eel, r1, r2);
+        } else if (c->o.kind_of_EMFP == 1) {
             dE = e_transfer_el(S, Eel, EMFPt, 0, id, &nd);
         } else {
             dE = 0.0;
@@ -1012,7 +1017,12 @@ static void hole_event(or_mc *S, int idx, double t_cur)
         } else {
             kind = 4;
             double HEt = or_nfp_1d(c, Eel, c->tab[TRK_TAB_H_EL].E, c->tab[TRK_TAB_H_EL].L[0], c->tab[TRK_TAB_H_EL].n);
-            if (c->o.kind_of_EMFP == 1) {
+            if (c->o.kind_of_EMFP == 2) {
+                /* NRG_transfer_elastic_DSF on the holes' tables, MC:2668-2677 */
+                double r1 = rn(S, id, &nd);
+                double r2 = rn(S, id, &nd);
+                dE = or_dsf_transfer(c, 1, Eel, r1, r2);
+            } else if (c->o.kind_of_EMFP == 1) {
                 dE = e_transfer_el(S, Eel, HEt, 1, id, &nd);
             } else {
                 dE = 0.0;

@@ -1202,6 +1202,7 @@ void or_destroy(or_ctx *c)
     free(c->dosE); free(c->dos); free(c->dosI); free(c->dosk); free(c->dosM); free(c->doskI); free(c->dosMI);
     free(c->El_IMFP); free(c->El_EMFP); free(c->Hole_IMFP); free(c->Hole_EMFP); free(c->Phot_IMFP); free(c->SHI_path);
     free(c->tim_glob);
+    or_dsf_free(c);
     free(c);
 }
 
@@ -1302,11 +1303,21 @@ int or_build_tables(or_ctx *c, int threads)
     th->nsh = nsh;
     /* elastic: AI:210 */
     or_table *tee = &c->tab[TRK_TAB_E_EL];
-    tee->E = grid_alloc(c, 0.01, c->e_emax, &tee->n);
-    tee->nsh = 1;
     or_table *the = &c->tab[TRK_TAB_H_EL];
-    the->E = grid_alloc(c, 0.01, dosmax + 1.0, &the->n);
+    tee->nsh = 1;
     the->nsh = 1;
+    if (c->o.kind_of_EMFP == 2) {
+        /* DSF: Elastic_MFP%Total on the DSF energies, AI:905-911, 1181-1184 */
+        if (c->dsf_nE[0] < 2 || c->dsf_nE[1] < 2) {
+            snprintf(c->err, sizeof c->err, "DSF elastic scattering needs the electron and hole DSF tables");
+            return 1;
+        }
+        or_dsf_elastic_table(c, 0, &tee->E, &tee->L[0], &tee->dEdx[0], &tee->n);
+        or_dsf_elastic_table(c, 1, &the->E, &the->L[0], &the->dEdx[0], &the->n);
+    } else {
+        tee->E = grid_alloc(c, 0.01, c->e_emax, &tee->n);
+        the->E = grid_alloc(c, 0.01, dosmax + 1.0, &the->n);
+    }
     /* photons */
     or_table *tp = &c->tab[TRK_TAB_PHOT];
     if (c->o.include_photons) {
@@ -1336,10 +1347,12 @@ int or_build_tables(or_ctx *c, int threads)
         }
     }
     th->rows[c->vb] = (or_row *)calloc((size_t)th->n, sizeof(or_row));
-    tee->L[0] = (double *)calloc((size_t)tee->n, sizeof(double));
-    tee->dEdx[0] = (double *)calloc((size_t)tee->n, sizeof(double));
-    the->L[0] = (double *)calloc((size_t)the->n, sizeof(double));
-    the->dEdx[0] = (double *)calloc((size_t)the->n, sizeof(double));
+    if (c->o.kind_of_EMFP != 2) {
+        tee->L[0] = (double *)calloc((size_t)tee->n, sizeof(double));
+        tee->dEdx[0] = (double *)calloc((size_t)tee->n, sizeof(double));
+        the->L[0] = (double *)calloc((size_t)the->n, sizeof(double));
+        the->dEdx[0] = (double *)calloc((size_t)the->n, sizeof(double));
+    }
     if (c->o.kind_of_EMFP == 1) {
         tee->rows[0] = (or_row *)calloc((size_t)tee->n, sizeof(or_row));
         the->rows[0] = (or_row *)calloc((size_t)the->n, sizeof(or_row));
@@ -1357,11 +1370,12 @@ int or_build_tables(or_ctx *c, int threads)
         int i = (int)(w / nsh), s = (int)(w % nsh);
         tot_imfp(c, th->E[i], s, 1, &th->L[s][i], &th->dEdx[s][i], (s == c->vb) ? &th->rows[s][i] : NULL);
     }
+    const int n_el = (c->o.kind_of_EMFP == 2) ? 0 : tee->n, n_hel = (c->o.kind_of_EMFP == 2) ? 0 : the->n;
 #pragma omp parallel for schedule(dynamic, 1)
-    for (int i = 0; i < tee->n; ++i)
+    for (int i = 0; i < n_el; ++i)
         elastic_cs(c, tee->E[i], 0, &tee->L[0][i], &tee->dEdx[0][i], tee->rows[0] ? &tee->rows[0][i] : NULL);
 #pragma omp parallel for schedule(dynamic, 1)
-    for (int i = 0; i < the->n; ++i)
+    for (int i = 0; i < n_hel; ++i)
         elastic_cs(c, the->E[i], 1, &the->L[0][i], &the->dEdx[0][i], the->rows[0] ? &the->rows[0][i] : NULL);
     if (tp->n) {
         for (int s = 0; s < nsh; ++s)

@@ -80,6 +80,8 @@ def test_file_names_follow_the_reference_rules():
     assert m["e_el"] == "OUTPUT_Electron_EMFPs_Mott.dat" and m["h_el"] == "OUTPUT_Hole_Mott_EMFPs_300.00_K.dat"
     m = _names(emfp=-1)
     assert m["e_el"] == "OUTPUT_Electron_No_elas_EMFPs.dat" and m["h_el"] == "OUTPUT_Hole_No_elas_EMFPs.dat"
+    m = _names(emfp=2)
+    assert m["e_el"] == "OUTPUT_Electron_DSF_EMFPs_300.00_K.dat" and m["h_el"] == "OUTPUT_Hole_DSF_EMFPs_300.00_K.dat"
     assert _names(ion_Z=79, kind_zeff=4)["shi"] == "OUTPUT_Au_CDF_fixed_P_IMFP.dat"
     # diff_CS rows: the file name without 'OUTPUT_' and '.dat', atom, shell, f14.3 energy
     assert cache._stem(n["e_inel"]) == "Electron_IMFPs_Free_CDF_Me_300.00_K"
@@ -93,7 +95,7 @@ class _OracleTables:
 
     def __init__(self, o):
         self.o = o
-        self.mat, self.ion, self.opt = o.mat, o.ion, o.opt
+        self.mat, self.ion, self.opt, self.desc = o.mat, o.ion, o.opt, o.desc
         self.tables, self.rows, self.finished = {}, {}, False
 
     def table(self, kind, shell=0):
@@ -176,3 +178,24 @@ def test_cache_of_another_grid_is_refused(oracle_cdf, tmp_path):
     with pytest.raises(cache.CacheMismatch, match="energy node 3"):
         cache.seed_from_cache(_OracleTables(o), str(tmp_path))
     assert math.isfinite(float(lines[0][:25]))
+
+
+def test_dsf_elastic_files_round_trip(lib, tmp_path):
+    # DSF elastic tables (AI:1392-1398): energy, total, emission and
+    # absorption paths; no diff_CS rows
+    from oracle import Oracle
+    mat, ion, opt, _ = tg.config_material("opt_dsf")
+    mat.prepare(lib)
+    o = Oracle(mat, ion, opt)
+    o.build_tables(threads=8, shi_window=(ion.E * 0.8, ion.E * 1.01))
+    names = cache.write_cache(_OracleTables(o), str(tmp_path))
+    for p, key in enumerate(("e_el", "h_el")):
+        a = np.array(cache._rows(str(tmp_path / names[key])))
+        t = mat.dsf[p]
+        assert a.shape == (len(t.E), 4)
+        assert np.array_equal(a[:, 2], t.dL_emit[:, -1]) and np.array_equal(a[:, 3], t.dL_absorb[:, -1])
+    dst = _OracleTables(o)
+    cache.seed_from_cache(dst, str(tmp_path))
+    for kind in (abi.TAB_E_EL, abi.TAB_H_EL):
+        assert np.array_equal(dst.tables[(kind, 0)][0], o.table(kind, 0)[1])
+    assert not any(k[0] in (abi.TAB_E_EL, abi.TAB_H_EL) for k in dst.rows)

@@ -175,16 +175,18 @@ def test_batch_autotuner_records_its_choice():
     assert sum(float(x["Out_tot_Ne"].sum()) for x in t) > 0
 
 
-def test_batch_refuses_the_delta_cdf_model():
-    """Impacts whose material uses the Delta-CDF model (kind_of_DR = 4) run
-    through trk_run_mc; a batch refuses them with the reason."""
+@pytest.mark.parametrize("name", ["opt_delta_cdf", "opt_dsf"])
+def test_batch_refuses_the_general_transfers(name):
+    """Impacts whose material uses the Delta-CDF model (kind_of_DR = 4) or
+    DSF elastic scattering run through trk_run_mc; a batch refuses them with
+    the reason."""
     lib = tg.load_library()
-    mat, ion, opt, run = tg.config_material("opt_delta_cdf")
+    mat, ion, opt, run = tg.config_material(name)
     mat.prepare(lib)
     ctx = tg.Context(mat, ion, opt)
     ctx.build_tables()
     c = IonContext(ctx, ion)
     c.build_tables()
     R, V = tg.radial_grid(run["NR"], run["Rmin"], run["Rmax"], 100.0)
-    with pytest.raises(tg.TrekisError, match="Delta-CDF"):
+    with pytest.raises(tg.TrekisError, match="batches do not"):
         Batch([c], R, V, capacity=1 << 16)

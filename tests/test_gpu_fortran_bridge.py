@@ -25,8 +25,9 @@ ORDER = ["Out_ne", "Out_Ee", "Out_nphot", "Out_Ephot", "Out_Ee_vs_E", "Out_Eh_vs
          "Out_theta", "Out_theta_h", "Out_diff_coeff", "Out_Ne_Em", "Out_E_Em", "Out_Ee_vs_E_Em"]
 
 
-def _descriptor(m, ion, opt, nmc, seed, Nt, R, V, ms):
-    """The driver's input stream (see bridge_driver.f90) from the C structs."""
+def _descriptor(m, ion, opt, nmc, seed, Nt, R, V, ms, dsf=None):
+    """The driver's input stream (see bridge_driver.f90) from the C structs
+    (and, with DSF elastic scattering, the electron and hole DSF tables)."""
     b = bytearray()
     i4 = lambda *v: b.extend(struct.pack(f"<{len(v)}i", *v))
     f8 = lambda *v: b.extend(struct.pack(f"<{len(v)}d", *v))
@@ -54,12 +55,18 @@ def _descriptor(m, ion, opt, nmc, seed, Nt, R, V, ms):
     f8(opt.Tim, opt.dt, opt.e_grid_Emax, opt.shi_grid_Emax)
     i4(nmc); b.extend(struct.pack("<q", seed)); i4(Nt, len(R), ms)
     f8(*R); f8(*V)
+    if opt.kind_of_EMFP == 2:
+        for t in dsf:
+            E, dE, dL, Le, La = t.arrays()
+            i4(*dE.shape)
+            for i in range(len(E)):
+                f8(E[i]); f8(*dE[i]); f8(*dL[i]); f8(*Le[i]); f8(*La[i])
     return bytes(b)
 
 
 @pytest.mark.skipif(not os.path.exists(DRIVER), reason="the Fortran bridge driver is built where the reference's "
                                                        "Objects.f90 is (make -C trekis-3_amd/fortran)")
-@pytest.mark.parametrize("name", ["test", "test_metal", "opt_emission", "opt_beb"])
+@pytest.mark.parametrize("name", ["test", "test_metal", "opt_emission", "opt_beb", "opt_dsf"])
 def test_fortran_do_monte_carlo_equals_c_abi(name, tmp_path):
     lib = tg.load_library()
     mat, ion, opt, run = tg.config_material(name)
@@ -73,7 +80,7 @@ def test_fortran_do_monte_carlo_equals_c_abi(name, tmp_path):
     m = ctx.mat
     ms = max(mat.n_shells)
     fin, fout = tmp_path / "in.bin", tmp_path / "out.bin"
-    fin.write_bytes(_descriptor(m, ion, opt, nmc, seed, ctx.Nt, R, V, ms))
+    fin.write_bytes(_descriptor(m, ion, opt, nmc, seed, ctx.Nt, R, V, ms, mat.dsf))
     say(f"[fortran] {name}: bridge driver")
     p = subprocess.run([DRIVER, str(fin), str(fout)], capture_output=True, text=True, timeout=240)
     say(p.stdout.strip(), p.stderr.strip()[-2000:])

@@ -13,7 +13,7 @@ import numpy as np
 import pytest
 
 import trekis_gpu as tg
-from trekis_gpu import abi
+from trekis_gpu import abi, dsf
 
 from gpu_helpers import setup, say, sort_trace, first_mismatch, compare_tallies
 
@@ -24,14 +24,15 @@ FULL = ["config1", "config2"]       # the Al-like/Xe and Al2O3-like/Au workloads
 # the option branches the workloads above leave at their defaults: dispersion
 # kinds 2 and 3, ion charge models 1-4, a 300 K target, the plasmon integration
 # limit, logarithmic time grids, a Brandt-Kitagawa ion, emission through the
-# surface, the Delta-CDF model and BEB shells (configs.OPTION_WORKLOADS)
+# surface, the Delta-CDF model, BEB shells and DSF elastic scattering
+# (configs.OPTION_WORKLOADS)
 OPTIONS = ["opt_plasmon_pole", "opt_ritchie_warm", "opt_sg_warm", "opt_fixed_zeff", "opt_bk_ion", "opt_emission",
-           "opt_delta_cdf", "opt_beb"]
+           "opt_delta_cdf", "opt_beb", "opt_dsf"]
 # table parity: test_sio2 has config5's tables (SiO2-like, U 20 MeV/u, phonon CDF, photons).
 # opt_ritchie_warm's full oracle tables take minutes (a deterministic pow per
 # q step); its rows are compared at sampled energies below instead.
 TABLES = ["test", "test_metal", "test_cdf", "test_polymer"] + FULL + ["config5"] + \
-    ["opt_plasmon_pole", "opt_sg_warm", "opt_fixed_zeff", "opt_bk_ion", "opt_delta_cdf", "opt_beb"]
+    ["opt_plasmon_pole", "opt_sg_warm", "opt_fixed_zeff", "opt_bk_ion", "opt_delta_cdf", "opt_beb", "opt_dsf"]
 
 
 @pytest.mark.parametrize("name", TABLES)
@@ -251,3 +252,30 @@ def test_error_paths():
     bad.precision = 3
     with pytest.raises(tg.TrekisError, match="invalid argument"):
         tg.Context(mat, ion, bad)
+
+
+def test_dsf_error_paths():
+    lib = tg.load_library()
+    mat, ion, opt, run = tg.config_material("opt_dsf")
+    mat.prepare(lib)
+    tables = mat.dsf
+    mat.dsf = None
+    ctx = tg.Context(mat, ion, opt)
+    # DSF elastic scattering without its tables
+    with pytest.raises(tg.TrekisError, match="DSF"):
+        ctx.build_tables()
+    with pytest.raises(tg.TrekisError, match="invalid"):
+        ctx.set_dsf(2, tables[0])
+    bad = dsf.DsfTable(tables[0].E[::-1].copy(), tables[0].dE, tables[0].dL, tables[0].dL_emit, tables[0].dL_absorb)
+    with pytest.raises(tg.TrekisError, match="increasing"):
+        ctx.set_dsf(0, bad)
+    ctx.set_dsf(0, tables[0])
+    ctx.set_dsf(1, tables[1])
+    ctx.build_tables()
+    with pytest.raises(tg.TrekisError, match="before trk_build_tables"):
+        ctx.set_dsf(0, tables[0])
+    # a context without DSF scattering takes no DSF tables
+    mat2, ion2, opt2, _ = tg.config_material("test")
+    mat2.prepare(lib)
+    with pytest.raises(tg.TrekisError, match="kind_of_EMFP = 2"):
+        tg.Context(mat2, ion2, opt2).set_dsf(0, tables[0])

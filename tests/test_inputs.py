@@ -260,3 +260,33 @@ def test_auger_times_the_reference_reads_from_eadl_are_refused():
             inputs.parse_cdf(text.replace("538.0e0\t2\t4.613e0", "538.0e0\t2\t" + bad))
     f = inputs.parse_cdf(text.replace("24\t1.0d23", "24\t-1.0"))
     assert f.atoms[0]["shells"][-1]["auger"] == 1.0e23
+
+
+def test_dsf_files_of_the_input_tree(tmp_path):
+    """kind_of_EMFP = 2 reads INPUT_DSF/<material>/<material>_{Electron,Hole}_
+    DSF_Differential_EMFPs_<T>K.dat (RF:316-322); without them the run falls
+    back to Mott scattering (RF:2543-2552)."""
+    import shutil
+    from trekis_gpu import dsf
+    src = os.path.join(os.path.dirname(os.path.abspath(configs.__file__)), "examples")
+    for d in ("INPUT_CDF", "INPUT_DOS"):
+        shutil.copytree(os.path.join(src, d), tmp_path / d)
+    lines = open(os.path.join(src, "INPUT_PARAMETERS.txt")).read().splitlines(True)
+    lines[8] = "300.0      ! [K] temperature of the target\n"
+    lines[11] = "2   0      ! elastic cross sections: 2=DSF\n"
+    par = tmp_path / "INPUT_PARAMETERS.txt"
+    par.write_text("".join(lines))
+    mat, ion, opt, _ = inputs.load_inputs(str(par), radiat="off")
+    assert opt.kind_of_EMFP == 0 and mat.dsf is None
+    folder = tmp_path / "INPUT_DSF" / "al2o3_like"
+    folder.mkdir(parents=True)
+    blocks = {}
+    for who, top in (("Electron", 1.0e4), ("Hole", 30.0)):
+        blocks[who] = dsf.synthetic_dsf_blocks(0.01, top, 300.0, nE=6, nW=40)
+        (folder / f"al2o3_like_{who}_DSF_Differential_EMFPs_300K.dat").write_text(dsf.dsf_file_text(*blocks[who]))
+    mat, ion, opt, _ = inputs.load_inputs(str(par), radiat="off")
+    assert opt.kind_of_EMFP == 2
+    for t, who in zip(mat.dsf, ("Electron", "Hole")):
+        want = dsf.process_dsf(*blocks[who])
+        for x, y in zip(t.arrays(), want.arrays()):
+            assert np.array_equal(x, y)

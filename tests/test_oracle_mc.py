@@ -22,7 +22,7 @@ import pytest
 import trekis_gpu as tg
 from oracle import Oracle, dp
 
-CONFIGS = ["test", "test_metal", "opt_emission", "opt_delta_cdf", "opt_beb"]
+CONFIGS = ["test", "test_metal", "opt_emission", "opt_delta_cdf", "opt_beb", "opt_dsf"]
 
 
 @pytest.fixture(scope="module", params=CONFIGS)

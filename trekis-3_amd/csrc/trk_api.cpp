@@ -56,7 +56,8 @@ static int validate(const trk_material *m, const trk_ion *ion, const trk_options
     }
     if (m->n_phonon < 0 || m->n_phonon > TRK_MAX_PHONON) { why = "n_phonon out of range"; return 1; }
     if (o->kind_of_EMFP == 1 && m->n_phonon < 1) { why = "phonon-CDF elastic scattering needs phonon oscillators"; return 1; }
-    if (o->kind_of_EMFP < -1 || o->kind_of_EMFP > 1) { why = "kind_of_EMFP must be -1, 0 or 1 (DSF is not built)"; return 1; }
+    if (o->kind_of_EMFP < -1 || o->kind_of_EMFP > 2)
+        { why = "kind_of_EMFP must be -1 (none), 0 (Mott), 1 (phonon CDF) or 2 (DSF)"; return 1; }
     if (m->n_dos < 2 || !m->dos_E || !m->dos_DOS || !m->dos_int_DOS || !m->dos_Eff_m) { why = "a VB density of states is required"; return 1; }
     if (m->El_eff_mass == 0.0 && (!m->dos_k || !m->dos_k_inv || !m->dos_Eff_m_inv)) { why = "El_eff_mass = 0 needs k and Eff_m arrays"; return 1; }
     if (m->work_function > 0.0) {
@@ -316,6 +317,36 @@ int trk_row_get(const trk_ctx *c, int kind, int shell, int32_t i_E, double *hw, 
 }
 
 // Tables replaced from host arrays: same grid, same row lengths as built.
+// DSF_DEMFP / DSF_DEMFP_H (Objects.f90:173-178) as the reference's reader
+// leaves them, kept on the host until trk_build_tables places them
+int trk_dsf_set(trk_ctx *c, int32_t particle, int32_t nE, int32_t nW, const double *E, const double *dE,
+                const double *dL, const double *dL_emit, const double *dL_absorb)
+{
+    if (!c) return TRK_ERR_ARG;
+    HostMat &h = c->hm;
+    if (c->base) return set_error(c, TRK_ERR_ARG, "an ion context shares its material's tables; set the DSF there");
+    if (h.o.kind_of_EMFP != 2)
+        return set_error(c, TRK_ERR_STATE, "the context does not scatter elastically on DSF tables (kind_of_EMFP = 2)");
+    if (c->tables_ready) return set_error(c, TRK_ERR_STATE, "DSF tables are set before trk_build_tables");
+    if ((particle != 0 && particle != 1) || nE < 2 || nW < 2 || !E || !dE || !dL || !dL_emit || !dL_absorb)
+        return set_error(c, TRK_ERR_ARG, "invalid trk_dsf_set arguments");
+    for (int i = 0; i < nE; ++i) {
+        if (!std::isfinite(E[i]) || !(E[i] > 0.0) || (i > 0 && !(E[i] > E[i - 1])))
+            return set_error(c, TRK_ERR_ARG, "DSF energies must be positive, finite and increasing (energy %d)", i);
+    }
+    const size_t cells = (size_t)nE * nW;
+    for (size_t k = 0; k < cells; ++k)
+        if (!std::isfinite(dE[k]) || dL[k] != dL[k] || dL_emit[k] != dL_emit[k] || dL_absorb[k] != dL_absorb[k])
+            return set_error(c, TRK_ERR_ARG, "DSF rows hold a NaN or an infinite transfer (entry %zu)", k);
+    h.dsf_nW[particle] = nW;
+    h.dsfE[particle].assign(E, E + nE);
+    h.dsfdE[particle].assign(dE, dE + cells);
+    h.dsfL[particle].assign(dL, dL + cells);
+    h.dsfLe[particle].assign(dL_emit, dL_emit + cells);
+    h.dsfLa[particle].assign(dL_absorb, dL_absorb + cells);
+    return 0;
+}
+
 static int check_settable(trk_ctx *c, int kind)
 {
     if (!c->tables_ready) return set_error(c, TRK_ERR_STATE, "trk_build_tables must run before tables are set");

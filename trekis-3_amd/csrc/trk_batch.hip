@@ -520,8 +520,8 @@ int trk_batch_create(trk_ctx *const *impacts, int32_t n, const double *Out_R, co
             return berr(b, TRK_ERR_STATE, "impact %d: tables were set, trk_tables_finish must run first", i);
         if (c->hm.o.precision != impacts[0]->hm.o.precision) return berr(b, TRK_ERR_ARG, "impacts mix precisions");
         if (uses_general_kernel(c->hm))
-            return berr(b, TRK_ERR_ARG, "impact %d uses the Delta-CDF model (kind_of_DR = 4) or BEB shells, which "
-                                        "trk_run_mc transports and batches do not", i);
+            return berr(b, TRK_ERR_ARG, "impact %d uses the Delta-CDF model (kind_of_DR = 4), BEB shells or DSF "
+                                        "elastic scattering, which trk_run_mc transports and batches do not", i);
         if (c->device != impacts[0]->device) return berr(b, TRK_ERR_ARG, "impacts on different devices");
         const trk_ctx *m = c->base ? c->base : c;
         auto it = gi.find(m);

@@ -23,6 +23,10 @@ struct HostMat {
     double shi_Zeff0 = 0;
     int Nt = 0;
     std::vector<double> tim_glob;
+    // DSF elastic tables (kind_of_EMFP = 2) of electrons [0] and holes [1]:
+    // nE energies x nW transfers, E-major
+    int dsf_nW[2] = {0, 0};
+    std::vector<double> dsfE[2], dsfdE[2], dsfL[2], dsfLe[2], dsfLa[2];
 };
 
 // Device copies of the material arrays behind MatDev.

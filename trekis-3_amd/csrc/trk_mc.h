@@ -146,6 +146,8 @@ struct McDev {
     const double *delta_alpha;
     const int *beb_kocs;
     const double *beb_Ek;
+    // DSF elastic scattering (kind_of_EMFP = 2) of electrons [0] and holes [1]
+    DsfDev dsf[2];
 };
 
 enum { EV_SHI = 0, EV_E_INEL, EV_E_EL, EV_H_INEL, EV_H_EL, EV_AUGER, EV_RAD, EV_PHOT };

@@ -466,6 +466,8 @@ void fill_mcdev(const trk_ctx *c, McDev &d)
     d.delta_alpha = mb.oAlpha;
     d.beb_kocs = mb.kocs;
     d.beb_Ek = mb.Ek;
+    d.dsf[0] = ts.dsf[0];
+    d.dsf[1] = ts.dsf[1];
     d.em_W = d.em_H = d.em_E1 = d.em_gamma = 0.0;
     const trk_material &m = c->hm.m;
     if (m.work_function > 0.0 &&
@@ -645,11 +647,12 @@ struct NoAllocBufs {
 static int finalize_on_device(trk_ctx *c, McState *s, const NoAllocBufs &na, const double *Out_V, int NR,
                               trk_tallies *out, bool want_spec);
 
-// the transport kernel instantiation with the general inelastic transfers
-// runs when a shell uses the BEB cross section or the Delta-CDF model is on
+// the transport kernel instantiation with the general transfers runs when a
+// shell uses the BEB cross section, the Delta-CDF model is on or elastic
+// scattering comes from DSF tables
 bool uses_general_kernel(const HostMat &h)
 {
-    if (h.o.kind_of_DR == 4) return true;
+    if (h.o.kind_of_DR == 4 || h.o.kind_of_EMFP == 2) return true;
     for (int s = 0; s < h.nsh; ++s)
         if (h.m.kocs[s] == 2) return true;
     return false;
@@ -675,7 +678,7 @@ static int run_core(trk_ctx *c, McState *s, int64_t nmc, uint64_t seed, int64_t 
 {
     auto tstart = std::chrono::steady_clock::now();
     const bool mixed = c->hm.o.precision == 1;   // FP32 elementary functions in the event code
-    const bool general = uses_general_kernel(c->hm);   // BEB shells or the Delta-CDF model
+    const bool general = uses_general_kernel(c->hm);   // BEB shells, the Delta-CDF model or DSF elastic scattering
     const HostMat &h = c->hm;
     const int Nt = h.Nt, nsh = h.nsh;
     hipStream_t st = c->stream;

@@ -5,6 +5,7 @@
 #include <vector>
 #include <string>
 #include "trk_dev_common.h"
+#include "trk_dsf.h"
 
 namespace trk {
 
@@ -94,6 +95,11 @@ struct TableSet {
     int64_t launches = 0;
     double build_seconds = 0.0;
     int64_t dcs_evals = 0;         // number of q-integrals evaluated
+    // DSF elastic scattering (kind_of_EMFP = 2) of electrons [0] and holes
+    // [1]: the sampling tables (their E and Lt alias the elastic tables') and
+    // the one allocation per particle behind dE, dLe, dLa, Le, La
+    DsfDev dsf[2] = {};
+    double *dsf_buf[2] = {nullptr, nullptr};
 };
 
 struct HostMat;  // defined in trk_ctx.h

@@ -844,7 +844,13 @@ void free_tables(TableSet &ts)
         }
         ts.el_imfp = ts.hole_imfp = ts.phot_imfp = nullptr;
         ts.ln_el_imfp = ts.ln_hole_imfp = ts.ln_Lel = ts.ln_LHel = nullptr;
+        ts.dsf[0] = ts.dsf[1] = DsfDev{};
         ts.shared_material = false;
+    }
+    for (int p = 0; p < 2; ++p) {
+        if (ts.dsf_buf[p]) (void)hipFree(ts.dsf_buf[p]);
+        ts.dsf_buf[p] = nullptr;
+        ts.dsf[p] = DsfDev{};
     }
     for (auto &t : ts.tab) {
         if (t.E) (void)hipFree(t.E);
@@ -898,6 +904,8 @@ static void borrow_material(TableSet &ts, const TableSet &mat)
     }
     ts.el_imfp = mat.el_imfp; ts.hole_imfp = mat.hole_imfp; ts.phot_imfp = mat.phot_imfp;
     ts.ln_el_imfp = mat.ln_el_imfp; ts.ln_hole_imfp = mat.ln_hole_imfp; ts.ln_Lel = mat.ln_Lel; ts.ln_LHel = mat.ln_LHel;
+    ts.dsf[0] = mat.dsf[0];
+    ts.dsf[1] = mat.dsf[1];
     ts.shared_material = true;
 }
 
@@ -917,6 +925,16 @@ int build_tables(const HostMat &h, const MatDev &md, TableSet &ts, hipStream_t s
     grid_4cs(h.m, 0.1, dosmax + 1.0, gH);
     grid_4cs(h.m, 0.01, h.e_emax, gEel);
     grid_4cs(h.m, 0.01, dosmax + 1.0, gHel);
+    if (h.o.kind_of_EMFP == 2) {
+        // the elastic tables of DSF scattering live on the DSF's own energies
+        // (Elastic_MFP%Total%E = DSF_DEMFP%E, Analytical_IMFPs.f90:905-911)
+        if (h.dsfE[0].size() < 2 || h.dsfE[1].size() < 2) {
+            err = "DSF elastic scattering (kind_of_EMFP = 2) needs the electron and hole DSF tables (trk_dsf_set)";
+            return TRK_ERR_STATE;
+        }
+        gEel = h.dsfE[0];
+        gHel = h.dsfE[1];
+    }
     if (h.o.include_photons) grid_4cs(h.m, 0.1, h.e_emax, gP);
     {
         double M = h.ion.Mass * kMp;
@@ -1100,8 +1118,38 @@ int build_tables(const HostMat &h, const MatDev &md, TableSet &ts, hipStream_t s
         }
         if (!beb_vals.empty()) HIP_OK(hipStreamSynchronize(st));   // the host values are then free to go
     }
+    // DSF elastic tables: the total mean free path is each row's last entry
+    // (its emission and absorption parts likewise), no stopping power; the
+    // rows go to the device for the transfer sampling
+    if (own && h.o.kind_of_EMFP == 2) {
+        for (int p = 0; p < 2; ++p) {
+            const int k = p ? TRK_TAB_H_EL : TRK_TAB_E_EL;
+            const int nE = (int)h.dsfE[p].size(), nW = h.dsf_nW[p];
+            const size_t cells = (size_t)nE * nW;
+            std::vector<double> host(3 * cells + 3 * (size_t)nE);
+            double *Lt = host.data() + 3 * cells, *Le = Lt + nE, *La = Le + nE;
+            for (int i = 0; i < nE; ++i) {
+                const size_t last = (size_t)i * nW + (nW - 1);
+                Lt[i] = h.dsfL[p][last];
+                Le[i] = h.dsfLe[p][last];
+                La[i] = h.dsfLa[p][last];
+            }
+            std::copy(h.dsfdE[p].begin(), h.dsfdE[p].end(), host.begin());
+            std::copy(h.dsfLe[p].begin(), h.dsfLe[p].end(), host.begin() + cells);
+            std::copy(h.dsfLa[p].begin(), h.dsfLa[p].end(), host.begin() + 2 * cells);
+            if (dalloc(&ts.dsf_buf[p], host.size(), err)) return TRK_ERR_DEVICE;
+            HIP_OK(hipMemcpy(ts.dsf_buf[p], host.data(), sizeof(double) * host.size(), hipMemcpyHostToDevice));
+            HIP_OK(hipMemcpy(T[k].L, Lt, sizeof(double) * nE, hipMemcpyHostToDevice));
+            HIP_OK(hipMemset(T[k].dEdx, 0, sizeof(double) * nE));
+            DsfDev &dv = ts.dsf[p];
+            dv.nE = nE; dv.nW = nW;
+            dv.E = T[k].E; dv.Lt = T[k].L;
+            dv.dE = ts.dsf_buf[p]; dv.dLe = ts.dsf_buf[p] + cells; dv.dLa = ts.dsf_buf[p] + 2 * cells;
+            dv.Le = ts.dsf_buf[p] + 3 * cells + nE; dv.La = dv.Le + nE;
+        }
+    }
     // Mott / disabled elastic tables
-    if (own && h.o.kind_of_EMFP != 1) {
+    if (own && h.o.kind_of_EMFP != 1 && h.o.kind_of_EMFP != 2) {
         k_mott<<<nblk(T[TRK_TAB_E_EL].n, 128), 128, 0, st>>>(md, T[TRK_TAB_E_EL].E, T[TRK_TAB_E_EL].n, 0,
                                                               T[TRK_TAB_E_EL].L, T[TRK_TAB_E_EL].dEdx);
         k_mott<<<nblk(T[TRK_TAB_H_EL].n, 128), 128, 0, st>>>(md, T[TRK_TAB_H_EL].E, T[TRK_TAB_H_EL].n, 1,

@@ -12,7 +12,8 @@
 ! Out_nh, Out_Eh, Out_Ehkin, Out_tot_Ne, Out_tot_Nphot, Out_tot_E, Out_E_e,
 ! Out_E_phot, Out_E_at, Out_E_h, Out_theta, Out_theta_h, Out_diff_coeff,
 ! Out_Ne_Em, Out_E_Em, Out_Ee_vs_E_Em in Fortran order, after the error flag
-! and number (two int32).
+! and number (two int32).  With DSF elastic scattering (kind_of_EMFP = 2) the
+! descriptor ends with the electron and hole DSF tables (read_dsf).
 program bridge_driver
    use iso_c_binding
    use Objects
@@ -95,6 +96,13 @@ program bridge_driver
    read(u) NMC, seed, Nt, NR, ms
    allocate(Out_R(NR), Out_V(NR))
    read(u) Out_R, Out_V
+   if (NumPar%kind_of_EMFP == 2) then
+      ! DSF_DEMFP and DSF_DEMFP_H as reading_DSF_cross_sections leaves them
+      call read_dsf(u, DSF_DEMFP)
+      call read_dsf(u, DSF_DEMFP_H)
+   else
+      allocate(DSF_DEMFP(0), DSF_DEMFP_H(0))
+   endif
    close(u)
    trk_bridge_seed = seed
    trk_bridge_e_grid_Emax = eg
@@ -117,7 +125,6 @@ program bridge_driver
    Error_message%Err = .false.
    Error_message%Err_Num = 0
    allocate(SHI_MFP(0), diff_SHI_MFP(0), Total_el_MFPs(0), Total_Hole_MFPs(0), Total_Photon_MFPs(0))
-   allocate(DSF_DEMFP(0), DSF_DEMFP_H(0))
 
    ! Universal_MC_for_SHI_MAIN.f90:271-276
    call do_Monte_Carlo(NMC, SHI, SHI_MFP, diff_SHI_MFP, Target_atoms, 1, Target_atoms(1)%N_shl, CDF_Phonon, &
@@ -135,4 +142,21 @@ program bridge_driver
    write(u) Out_diff_coeff
    write(u) Out_Ne_Em, Out_E_Em, Out_Ee_vs_E_Em
    close(u)
+
+contains
+
+   ! one particle's DSF_DEMFP: nE, nW (int32), then per energy E and the
+   ! rows dE, dL, dL_emit, dL_absorb (nW doubles each)
+   subroutine read_dsf(u, D)
+      integer, intent(in) :: u
+      type(Differential_MFP), dimension(:), allocatable, intent(inout) :: D
+      integer(c_int32_t) :: nE, nW
+      integer :: i
+      read(u) nE, nW
+      allocate(D(nE))
+      do i = 1, nE
+         allocate(D(i)%dE(nW), D(i)%dL(nW), D(i)%dL_emit(nW), D(i)%dL_absorb(nW))
+         read(u) D(i)%E, D(i)%dE, D(i)%dL, D(i)%dL_emit, D(i)%dL_absorb
+      enddo
+   end subroutine read_dsf
 end program bridge_driver

@@ -241,6 +241,35 @@ contains
       Error_message%Err_descript = trim(where)//': '//trim(msg)
    end subroutine report
 
+   ! One particle's DSF tables (Differential_MFP, Objects.f90:173-178) into
+   ! the library: energies and the E-major rows of dE, dL, dL_emit, dL_absorb.
+   ! An empty or ragged set is refused by the library's checks.
+   integer(c_int) function pass_dsf(ctx, particle, D) result(rc)
+      type(c_ptr), intent(in) :: ctx
+      integer, intent(in) :: particle
+      type(Differential_MFP), dimension(:), intent(in) :: D
+      real(c_double), dimension(:), allocatable :: E, dE, dL, Le, La
+      integer :: nE, nW, i
+      nE = size(D)
+      nW = 0
+      if (nE > 0) nW = size(D(1)%dE)
+      do i = 1, nE
+         if (size(D(i)%dE) /= nW .or. size(D(i)%dL) /= nW .or. size(D(i)%dL_emit) /= nW .or. &
+             size(D(i)%dL_absorb) /= nW) nW = 0
+      enddo
+      allocate(E(max(nE, 1)), dE(max(nE * nW, 1)), dL(max(nE * nW, 1)), Le(max(nE * nW, 1)), La(max(nE * nW, 1)))
+      do i = 1, nE
+         E(i) = D(i)%E
+         if (nW > 0) then
+            dE((i - 1) * nW + 1 : i * nW) = D(i)%dE
+            dL((i - 1) * nW + 1 : i * nW) = D(i)%dL
+            Le((i - 1) * nW + 1 : i * nW) = D(i)%dL_emit
+            La((i - 1) * nW + 1 : i * nW) = D(i)%dL_absorb
+         endif
+      enddo
+      rc = trk_dsf_set(ctx, int(particle, c_int32_t), int(nE, c_int32_t), int(nW, c_int32_t), E, dE, dL, Le, La)
+   end function pass_dsf
+
    ! do_Monte_Carlo with the reference's argument list (Monte_Carlo.f90:39-92).
    subroutine do_Monte_Carlo(NMC, SHI, SHI_MFP, diff_SHI_MFP, Target_atoms, Lowest_Ip_At, Lowest_Ip_Shl, CDF_Phonon, &
         Total_el_MFPs, Elastic_MFP, Total_Hole_MFPs, Elastic_Hole_MFP, Total_Photon_MFPs, Mat_DOS, Tim, dt, Matter, &
@@ -314,6 +343,16 @@ contains
          call report(ctx, rc, 'trk_ctx_create', Error_message)
          call trk_ctx_destroy(ctx)
          return
+      endif
+      if (NumPar%kind_of_EMFP == 2) then
+         ! DSF_DEMFP / DSF_DEMFP_H as reading_DSF_cross_sections left them
+         rc = pass_dsf(ctx, 0, DSF_DEMFP)
+         if (rc == 0) rc = pass_dsf(ctx, 1, DSF_DEMFP_H)
+         if (rc /= 0) then
+            call report(ctx, rc, 'trk_dsf_set', Error_message)
+            call trk_ctx_destroy(ctx)
+            return
+         endif
       endif
       rc = trk_build_tables(ctx)
       if (rc /= 0) then

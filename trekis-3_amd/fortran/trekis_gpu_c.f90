@@ -83,6 +83,13 @@ module trekis_gpu_c
          character(kind=c_char), intent(out) :: buf(*)
          integer(c_size_t), value :: n
       end function
+      integer(c_int) function trk_dsf_set(ctx, particle, nE, nW, E, dE, dL, dL_emit, dL_absorb) &
+            bind(C, name="trk_dsf_set")
+         import :: c_int, c_ptr, c_int32_t, c_double
+         type(c_ptr), value :: ctx
+         integer(c_int32_t), value :: particle, nE, nW
+         real(c_double), intent(in) :: E(*), dE(*), dL(*), dL_emit(*), dL_absorb(*)
+      end function
       integer(c_int) function trk_build_tables(ctx) bind(C, name="trk_build_tables")
          import :: c_int, c_ptr
          type(c_ptr), value :: ctx

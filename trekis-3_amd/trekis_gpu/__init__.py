@@ -60,9 +60,24 @@ class Context:
             msg = self.error()
             self.close()
             raise TrekisError(f"trk_ctx_create: {abi.STATUS.get(rc, rc)}: {msg}")
+        if options.kind_of_EMFP == 2 and getattr(material, "dsf", None) is not None:
+            try:
+                for particle, t in enumerate(material.dsf):
+                    self.set_dsf(particle, t)
+            except TrekisError:
+                self.close()
+                raise
         nt = ctypes.c_int32()
         self.lib.trk_time_nodes(self.h, ctypes.byref(nt))
         self.Nt = nt.value
+
+    def set_dsf(self, particle, table):
+        """DSF elastic tables (dsf.DsfTable) of electrons (particle 0) or
+        holes (1); before build_tables."""
+        E, dE, dL, Le, La = table.arrays()
+        nE, nW = dE.shape
+        self._check(self.lib.trk_dsf_set(self.h, particle, nE, nW, dptr(E), dptr(dE), dptr(dL), dptr(Le), dptr(La)),
+                    "trk_dsf_set")
 
     def error(self):
         if not self.h:

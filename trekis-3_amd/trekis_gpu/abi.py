@@ -142,6 +142,7 @@ def declare_library(lib):
         "trk_row_size": (c_int, [c_void_p, c_int, c_int, c_int32, P(c_int32)]),
         "trk_row_get": (c_int, [c_void_p, c_int, c_int, c_int32, DP, DP, c_int32]),
         "trk_table_set": (c_int, [c_void_p, c_int, c_int, DP, DP, DP, c_int32]),
+        "trk_dsf_set": (c_int, [c_void_p, c_int32, c_int32, c_int32, DP, DP, DP, DP, DP]),
         "trk_row_set": (c_int, [c_void_p, c_int, c_int, c_int32, DP, DP, c_int32]),
         "trk_tables_finish": (c_int, [c_void_p]),
         "trk_time_nodes": (c_int, [c_void_p, P(c_int32)]),

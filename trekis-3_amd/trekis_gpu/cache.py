@@ -184,6 +184,10 @@ def file_names(mat, ion, opt):
     if opt.kind_of_EMFP == 1:
         n["e_el"] = f"OUTPUT_Electron_EMFPs{kcs}_{_ZEFF_TAG.get(int(opt.CDF_elast_Zeff), 'Zeff_')}{_temp(T, 7)}_K.dat"
         n["h_el"] = f"OUTPUT_Hole{kcs}_EMFPs_{_temp(T, 7)}_K.dat"
+    elif opt.kind_of_EMFP == 2:
+        # AI:809-811, 1142-1145
+        n["e_el"] = f"OUTPUT_Electron_DSF_EMFPs_{_temp(T, 7)}_K.dat"
+        n["h_el"] = f"OUTPUT_Hole_DSF_EMFPs_{_temp(T, 7)}_K.dat"
     elif opt.kind_of_EMFP == 0:
         n["e_el"] = "OUTPUT_Electron_EMFPs_Mott.dat"
         n["h_el"] = f"OUTPUT_Hole_Mott_EMFPs_{_temp(T, 7)}_K.dat"
@@ -285,11 +289,18 @@ def write_cache(ctx, folder, ion_only=False):
     if ctx.opt.include_photons:
         E, L, _ = _kind_tables(ctx, abi.TAB_PHOT, nsh)
         _write_mfp(os.path.join(folder, names["phot"]), E, L)
-    for kind, key in ((abi.TAB_E_EL, "e_el"), (abi.TAB_H_EL, "h_el")):
+    for p, (kind, key) in enumerate(((abi.TAB_E_EL, "e_el"), (abi.TAB_H_EL, "h_el"))):
         E, L, _ = ctx.table(kind, 0)
         with open(os.path.join(folder, names[key]), "w") as f:
-            for i in range(len(E)):
-                f.write(fortran_f(E[i]) + fortran_e(L[i]) + "\n")
+            if ctx.opt.kind_of_EMFP == 2:
+                # DSF: total, emission and absorption paths (AI:1392-1398)
+                t = ctx.desc.dsf[p]
+                for i in range(len(E)):
+                    f.write(fortran_f(E[i]) + fortran_es(L[i]) + fortran_es(t.dL_emit[i, -1])
+                            + fortran_es(t.dL_absorb[i, -1]) + "\n")
+            else:
+                for i in range(len(E)):
+                    f.write(fortran_f(E[i]) + fortran_e(L[i]) + "\n")
     for kind, s, i, path in _row_files(ctx, names, folder):
         hw, lam = ctx.row(kind, s, i)
         with open(path, "w") as f:
@@ -352,8 +363,12 @@ def seed_from_cache(ctx, folder, ion_only=False):
             E, _, _ = ctx.table(kind, 0)
             path = os.path.join(folder, names[key])
             a = np.array(_rows(path))
-            if a.ndim != 2 or a.shape[1] != 2:
-                raise CacheMismatch(f"{path}: rows must hold the energy and the mean free path")
+            # a DSF file adds the emission and absorption paths, which the
+            # library takes from the DSF tables' last entries, as they are made
+            ncol = 4 if ctx.opt.kind_of_EMFP == 2 else 2
+            if a.ndim != 2 or a.shape[1] != ncol:
+                raise CacheMismatch(f"{path}: rows must hold the energy and the mean free path"
+                                    + (" (total, emission, absorption)" if ncol == 4 else ""))
             _check_grid(path, a[:, 0], E, 0.0)
             ctx.set_table(kind, 0, L=a[:, 1])
         for kind, s, i, path in _row_files(ctx, names, folder):

@@ -71,6 +71,9 @@ class MaterialDesc:
     phonon_single_pole: bool = False
     dos_raw_E: np.ndarray = None
     dos_raw: np.ndarray = None
+    # (electron, hole) DSF tables (dsf.DsfTable) of DSF elastic scattering
+    # (kind_of_EMFP = 2); handed to the library when a context is created
+    dsf: tuple = None
 
     def __post_init__(self):
         self.n_atoms = len(self.atoms)
@@ -425,6 +428,10 @@ def config_material(name):
             shells = [sh for a in mat.atoms for sh in a.shells]
             for s, Ek in rest[2].items():
                 shells[s].kocs, shells[s].Ek = 2, Ek
+        if len(rest) > 3 and rest[3]:
+            from .dsf import synthetic_dsf
+            opt.kind_of_EMFP, opt.CDF_elast_Zeff = 2, 0
+            mat.dsf = synthetic_dsf(temp)
         if dr == 4:
             # electrons slowed below the peaks scatter elastically for the
             # rest of the window: ten times the events per iteration
@@ -475,6 +482,7 @@ OPTION_WORKLOADS = {
     "opt_emission": ("test", 1, 0, 8.0, 0.0, 0, 0, 1.0, 0, (4.5, 10.0, 10.0)),
     "opt_delta_cdf": ("test", 4, 0, 8.0, 0.0, 0, 0, 1.0),
     "opt_beb": ("test", 1, 0, 8.0, 0.0, 0, 0, 1.0, 0, None, {0: 105.0, 2: 700.0}),
+    "opt_dsf": ("test", 1, 0, 8.0, 300.0, 0, 0, 1.0, 0, None, {}, True),
 }
 
 

@@ -347,9 +347,11 @@ def load_inputs(parameters_path: str, root: str = None, radiat=None):
     """(MaterialDesc, Ion, Options, NMC) from the reference's input tree: the
     parameter file and, under root (default: the parameter file's folder),
     INPUT_CDF/<material>.cdf and INPUT_DOS/<material>.dos (or the files the
-    parameter file names).  Options this library does not compute (DSF
-    elastic tables, Delta-CDF, BEB) are refused by the library when the
-    context is created.
+    parameter file names).  With DSF elastic scattering (kind_of_EMFP = 2)
+    the electron and hole DSF files INPUT_DSF/<material>/<material>_{Electron,
+    Hole}_DSF_Differential_EMFPs_<T>K.dat (RF:316-322) are read into
+    MaterialDesc.dsf; when one is missing the run falls back to Mott
+    scattering, as the reference does (RF:2543-2552).
 
     radiat: radiative decay times [fs] per (element, shell), which the
     reference reads from the EADL database (Dealing_with_EADL.f90:356).  With
@@ -377,4 +379,13 @@ def load_inputs(parameters_path: str, root: str = None, radiat=None):
     opt.kind_of_EMFP, opt.CDF_elast_Zeff = p["kind_of_EMFP"], p["CDF_elast_Zeff"]
     opt.kind_of_DR, opt.plasmon_Emax, opt.include_photons = p["kind_of_DR"], p["plasmon_Emax"], p["include_photons"]
     opt.Tim, opt.dt, opt.dt_flag = p["Tim"], p["dt"], p["dt_flag"]
+    if opt.kind_of_EMFP == 2:
+        from .dsf import read_dsf
+        name = p["material"]
+        files = [os.path.join(root, "INPUT_DSF", name, f"{name}_{who}_DSF_Differential_EMFPs_{int(mat.temp)}K.dat")
+                 for who in ("Electron", "Hole")]
+        if all(os.path.exists(f) for f in files):
+            mat.dsf = tuple(read_dsf(f) for f in files)
+        else:
+            opt.kind_of_EMFP = 0
     return mat, ion, opt, p["NMC"]
