@@ -181,3 +181,40 @@ def test_photon_paths_from_cross_sections():
         if above.sum() > 2:
             e1, e2 = E[above][0], E[above][-1]
             assert L[s][above][-1] / L[s][above][0] == pytest.approx((e2 / e1) ** 3, rel=1e-12)
+
+
+def test_parameter_echo_follows_print_parameters(lib, tmp_path):
+    """'!Parameters.txt' (print_parameters, SO:42-337; Save_output SO:489-506):
+    the option lines for the run's models, and the atomic table, whose k-sum
+    of every core shell is its electron count (the oscillators are normalised
+    by that sum rule with the same plasma frequency, CS:600-613)."""
+    import datetime
+    import trekis_gpu as tg
+    from trekis_gpu import output
+    mat, ion, opt, run = tg.config_material("opt_fixed_zeff")
+    mat.prepare(lib)
+    t = tg.TallySet(11, 4, mat)
+    t["Out_tot_E"][:] = np.linspace(0.0, 5000.0, 11)
+    name = output.write_parameters(str(tmp_path), mat, ion, opt, 4, lib, tallies=t, zeff=3.5, duration=3725.25,
+                                   cdf_file="x.cdf", when=datetime.datetime(2026, 1, 2, 3, 4, 5, 6000))
+    lines = open(tmp_path / name).read().splitlines()
+    assert lines[0] == "*" * 56 and lines[-1] == "-" * 56
+    assert " Ion fixed charge 3.500 [e] is used with 3.500 model for SHI." in lines
+    assert " Inelastic scattering calculated with free electrons" in lines
+    assert " Elastic scattering calculated with phonon CDF (Ritchie-Howie)" in lines
+    assert " starting with 0.01 [fs] increasing by " + f"{opt.Tim:.3f}" + " in logarithmic time-scale" in lines
+    assert " Temperature of the target: 300.000 [K]" in lines
+    assert " CDF file used: x.cdf" in lines
+    assert "Printed at 03:04:05:006 on 02/01/2026" in lines
+    assert "MC calculated energy loss (Se): " + f"{5000.0 / 4 / mat.layer:.2f}" + " [eV/A]" in lines
+    assert "Duration of calculation:        1 hour 2 mins 5.250 sec" in lines
+    rows = [ln for ln in lines if ln[:11].rstrip().endswith("-shell")]
+    core = [sh for ia, a in enumerate(mat.atoms) for js, sh in enumerate(a.shells)
+            if not (ia == 0 and js == len(a.shells) - 1)]
+    assert len(rows) == len(core)
+    for ln, sh in zip(rows, core):
+        assert float(ln[11 + 3 + 8 + 10 + 10 + 24:][:9]) == pytest.approx(sh.Nel, abs=0.006), ln
+        assert float(ln[14:22]) == sh.Nel and float(ln[22:32]) == pytest.approx(sh.Ip, abs=0.006)
+    assert sum(ln.startswith("Valence") for ln in lines) == 1
+    assert output.duration_text(59.5) == "59.500 sec"
+    assert output.duration_text(2 * 86400 + 7200 + 61.0) == "2 days 2 hours 1 min 1.000 sec"

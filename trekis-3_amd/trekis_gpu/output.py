@@ -12,6 +12,8 @@ sum over the time nodes so far (SO:931) and as a temperature per atom
 node, as the files do.  write_output() writes the files themselves with
 the reference's names, headers and Fortran fields.
 """
+import math
+
 import numpy as np
 
 # Universal_Constants.f90:39, Boltzmann constant [K/eV]
@@ -123,7 +125,7 @@ def write_output(folder, tallies, nmc, mat, time_grid, Out_R, include_photons, d
     E25.16); with emission on (work_function > 0, default the material's)
     also the emitted electrons' spectrum.  Quantities of the field model,
     which this library does not build, are written as zeros where the files
-    hold them.  The parameter echo '!Parameters.txt' is not written.
+    hold them.  The parameter echo '!Parameters.txt' is write_parameters'.
     Returns the file names."""
     import os
     os.makedirs(folder, exist_ok=True)
@@ -230,3 +232,204 @@ def write_output(folder, tallies, nmc, mat, time_grid, Out_R, include_photons, d
                 _matrix_file(path(f"Radial_{_shell_tag(atom, sh)}_holes_temperature[K].txt"), "#Radius[A] ", times,
                              R, temp.T[:, :N])
     return written
+
+
+# Element names as Find_element_name gives them (Dealing_with_EADL.f90:
+# 1217-1703), printed as the ion's full name in the parameter echo.
+ELEMENT_NAME = (
+    None, "Hydrogen", "Helium", "Lithium", "Beryllium", "Boron", "Carbon", "Nitrogen", "Oxygen", "Fluorine", "Neon",
+    "Sodium", "Magnesium", "Aluminium", "Silicon", "Phosphorus", "Sulfur", "Chlorine", "Argon", "Potassium",
+    "Calcium", "Scandium", "Titanium", "Vanadium", "Chromium", "Manganese", "Iron", "Cobalt", "Nickel", "Copper",
+    "Zinc", "Gallium", "Germanium", "Arsenic", "Selenium", "Bromine", "Krypton", "Rubidium", "Strontium", "Yttrium",
+    "Zirconium", "Niobium", "Molybdenum", "Technetium", "Ruthenium", "Rhodium", "Palladium", "Silver", "Cadmium",
+    "Indium", "Tin", "Antimony", "Tellurium", "Iodine", "Xenon", "Caesium", "Barium", "Lanthanum", "Cerium",
+    "Praseodymium", "Neodymium", "Promethium", "Samarium", "Europium", "Gadolinium", "Terbium", "Dysprosium",
+    "Holmium", "Erbium", "Thulium", "Ytterbium", "Lutetium", "Hafnium", "Tantalum", "Tungsten", "Rhenium", "Osmium",
+    "Iridium", "Platinum", "Gold", "Mercury", "Thallium", "Lead", "Bismuth", "Polonium", "Astatine", "Radon",
+    "Francium", "Radium", "Actinium", "Thorium", "Protactinium", "Uranium", "Neptunium", "Plutonium", "Americium",
+    "Curium", "Berkelium", "Californium", "Einsteinium", "Fermium", "Mendelevium", "Nobelium", "Lawrencium",
+    "Rutherfordium", "Dubnium", "Seaborgium", "Bohrium", "Hassium", "Meitnerium", "Darmstadtium", "Roentgenium",
+    "Copernicium", "Ununtrium", "Flerovium", "Ununpentium", "Livermorium", "Ununseptium", "Ununoctium",
+)
+
+_DASHES = "-" * 56
+_STARS = "*" * 56
+
+
+def duration_text(sec):
+    """parse_time (SO:1439-1484): days, hours and minutes above their
+    counts, then the seconds as f7.3."""
+    days = hours = mins = 0.0
+    if sec >= 60.0:
+        mins = math.floor(sec / 60.0)
+        sec = sec - mins * 60.0
+        if mins > 60.0:
+            hours = math.floor(mins / 60.0)
+            mins = mins - hours * 60.0
+            if hours > 24.0:
+                days = math.floor(hours / 24.0)
+                hours = hours - days * 24.0
+    out = ""
+    for v, one, many in ((days, " day", " days"), (hours, " hour", " hours"), (mins, " min", " mins")):
+        if v > 1.0:
+            out = (out + " " + str(int(v)) + many).strip()
+        elif v > 0.0:
+            out = (out + " " + str(int(v)) + one).strip()
+    return (out + " " + f"{sec:7.3f}".strip() + " sec").strip()
+
+
+def parameter_lines(mat, ion, opt, nmc, lib, material_name=None, target_name=None, dos_file="", cdf_file="",
+                    when=None):
+    """The lines print_parameters writes (Sorting_output_data.f90:42-337)
+    for the run of material desc mat (prepared), ion and options: ion,
+    material, time grid, models, options, and the atomic table with every
+    shell's k- and f-sum (sumrules, with the plasma frequency of the shell's
+    atoms).  Without the title block (print_title false); the thread line
+    names the GPU instead of OpenMP threads.  when: the time stamp
+    (datetime, default now)."""
+    import ctypes
+    import datetime
+    from . import abi
+    from .cache import ELEMENT_SYMBOL, fortran_e, fortran_es, shell_name
+    from .configs import G_MP
+
+    def f(x, w, d):
+        return f"{x:{w}.{d}f}"
+
+    def es(x, w, d):
+        return fortran_es(float(x), w, d + 1)
+
+    name = material_name or mat.name
+    L = [_STARS]
+    if ion.Zat > 0:
+        L.append(f" Performing calculations for {ELEMENT_NAME[ion.Zat]} in {name}")
+        L.append(f" Ion {ELEMENT_SYMBOL[ion.Zat]} (Element #{ion.Zat}, Mass {f(ion.Mass, 12, 3).strip()})")
+        L.append(f" With energy {f(ion.E / 1.0e6, 9, 2).strip()} [MeV]")
+    else:
+        L.append(f" Performing calculations of the electron MFP in {name} only.")
+    L.append(_DASHES)
+    L.append(f" Material: {name} ({target_name or mat.name})")
+    L.append(f" Material density: {f(mat.dens, 12, 3).strip()} [g/cm^3] or {fortran_e(mat.At_Dens, 16, 7).strip()}"
+             " [1/cm^3]")
+    L.append(f" Thickness of the analysed layer: {f(mat.layer, 12, 3).strip()} [A]")
+    L.append(f" Temperature of the target: {f(mat.temp, 12, 3).strip()} [K]")
+    if mat.el_eff_mass == 0:
+        L.append(" Effective mass of valence electrons is calculated from DOS.")
+    else:
+        v = f(mat.el_eff_mass, 10, 2) if abs(mat.el_eff_mass) < 1e6 else fortran_e(mat.el_eff_mass, 16, 7)
+        L.append(f" Effective mass of valence electrons: {v.strip()} [me]")
+    if mat.hole_mass > 0:
+        v = f(mat.hole_mass, 10, 2) if abs(mat.hole_mass) < 1e6 else fortran_e(mat.hole_mass, 16, 7)
+        L.append(f" Effective mass of valence holes: {v.strip()} [me]")
+    else:
+        L.append(" Effective mass of valence holes is calculated from DOS")
+    L.append(f" DOS file used: {dos_file}" if dos_file else " No DOS file was found, not even free-electron one!")
+    L.append(f" CDF file used: {cdf_file}")
+    L.append(_DASHES)
+    if ion.Zat > 0:
+        tim = f(opt.Tim, 12, 3).strip()
+        L.append(f" Total time to be analysed: {tim} [fs]")
+        if opt.dt_flag <= 0:
+            L.append(f" with the timestep of {f(opt.dt, 12, 3).strip()} [fs], linear time-scale")
+        else:
+            # SO:141 prints the text last formatted, the total time, as the ratio
+            L.append(f" starting with 0.01 [fs] increasing by {tim} in logarithmic time-scale")
+    L.append(_DASHES)
+    model = "Brandt-Kitagawa" if ion.Kind_ion == 1 else "point-like charge"
+    if ion.Kind_Zeff == 4:
+        # SO:157-158 prints the charge where the model's name would go
+        z = f(ion.fixed_Zeff, 7, 3).strip()
+        L.append(f" Ion fixed charge {z} [e] is used with {z} model for SHI.")
+    else:
+        formula = {1: "Bohr", 2: "Nikolaev-Dmitriev", 3: "Schiwietz-Grande"}.get(int(ion.Kind_Zeff), "Barkas")
+        L.append(f" Ion equilibrium charge is used with {formula} formula, {model} model for SHI.")
+    if max(sh.kocs for sh in mat.atoms[0].shells) == 1:
+        dr = {3: "dispersion relation from Ritchie and Howie", 2: "plasmon-pole dispersion relation of scattering "
+              "centers", 4: "Delta-function CDF"}.get(int(opt.kind_of_DR), "free electrons")
+    else:
+        dr = "BEB atomic cross sections"
+    L.append(f" Inelastic scattering calculated with {dr}")
+    if opt.plasmon_Emax:
+        L.append(" Plasmon maximal energy is used as upper integration limit during cross-section calculation")
+    L.append(" (using new integration energy grid)")
+    el = {2: "calculated with DSF cross-sections", 1: "calculated with phonon CDF (Ritchie-Howie)",
+          0: "calculated with Mott atomic cross-sections"}.get(int(opt.kind_of_EMFP), "is excluded")
+    L.append(f" Elastic scattering {el}")
+    if mat.cut_off > 0.0:
+        L.append(f" Energy cut-off used is {f(mat.cut_off, 12, 3).strip()}")
+    else:
+        L.append(" No energy cut-off is used")
+    if mat.work_function > 0:
+        L.append(f" Electron emission included. Work function = {f(mat.work_function, 12, 3).strip()} [eV]")
+        L.append(f" Potential barrier length = {f(mat.bar_length, 12, 3).strip()} [A] Barrier height = "
+                 f"{f(mat.bar_height, 12, 3).strip()} [eV]")
+    else:
+        L.append(" Electron emission is excluded")
+    L.append(" Radiative decays of deep-shell holes and photon transport are "
+             + ("included" if opt.include_photons else "excluded"))
+    L.append(" Transient electric fields are excluded")
+    L.append(f" Number of MC iterations: {f'{int(nmc):5d}'.strip()}")
+    L.append(" Transport on 1 GPU")
+    # atomic parameters (SO:276-330)
+    n_at_mol = sum(a.pers for a in mat.atoms)
+    L.append(_DASHES)
+    L.append(" The following atomic parameters of the target are used:")
+
+    def sums(osc, Ip, Omega):
+        if not osc:
+            return 0.0, 0.0
+        A = np.ascontiguousarray([o[1] for o in osc], dtype=np.float64)
+        E0 = np.ascontiguousarray([o[0] for o in osc], dtype=np.float64)
+        G = np.ascontiguousarray([o[2] for o in osc], dtype=np.float64)
+        ks, fs = ctypes.c_double(), ctypes.c_double()
+        lib.trk_sumrules(A.ctypes.data_as(abi.DP), E0.ctypes.data_as(abi.DP), G.ctypes.data_as(abi.DP), len(A), Ip,
+                         Omega, ctypes.byref(ks), ctypes.byref(fs))
+        return ks.value, fs.value
+
+    for ia, atom in enumerate(mat.atoms):
+        L.append(f"{ELEMENT_SYMBOL[atom.Z]} atom:")
+        L.append(" Shell  Quantum_n  Ne    Ip[eV]  Ekin[eV]  t(Auger)[fs]  t(Rad)[fs]   k-sum    f-sum")
+        for js, sh in enumerate(atom.shells):
+            vb = ia == 0 and js == len(atom.shells) - 1
+            dens = 1e6 * mat.At_Dens / n_at_mol if vb else 1e6 * mat.At_Dens * atom.pers / n_at_mol
+            ks, fs = sums(sh.osc, sh.Ip, lib.trk_w_plasma(dens, 0.0))
+            nm = ("Valence" if vb else shell_name(sh.shl_num))[:11].ljust(11)
+            pqn = "   " if vb else f"{'KLMNOPQ'.index(nm[0]) + 1 if nm[0] in 'KLMNOPQ' else 0:3d}"
+            L.append(nm + pqn + f(sh.Nel, 8, 2) + f(sh.Ip, 10, 2) + f(sh.Ek, 10, 2) + es(sh.auger, 12, 2)
+                     + es(sh.radiat, 12, 2) + f(ks, 9, 2) + f(fs, 9, 2))
+    mean_mass = sum(a.pers * a.mass for a in mat.atoms) * G_MP / n_at_mol
+    ks, fs = sums([(e0, a, g) for e0, a, g in mat.phonon], 1.0e-8,
+                  lib.trk_w_plasma(1e6 * mat.At_Dens / n_at_mol, mean_mass))
+    if ks > 1.0e-2:
+        L.append("Phonons" + "       " + f(n_at_mol, 8, 2) + f(0.0, 10, 2) + f(0.0, 10, 2) + es(0.0, 12, 2)
+                 + es(0.0, 12, 2) + f(ks, 9, 2) + f(fs, 9, 2))
+    else:
+        L.append("Phonons" + "       " + f(n_at_mol, 8, 2) + f(0.0, 9, 2) + f(0.0, 9, 2) + es(0.0, 12, 2)
+                 + es(0.0, 12, 2) + es(ks, 12, 2) + es(fs, 12, 2))
+    L.append(_DASHES)
+    t = when or datetime.datetime.now()
+    L.append(f"Printed at {t.hour:02d}:{t.minute:02d}:{t.second:02d}:{t.microsecond // 1000:03d} on "
+             f"{t.day:02d}/{t.month:02d}/{t.year:04d}")
+    L.append(_STARS)
+    return L
+
+
+def write_parameters(folder, mat, ion, opt, nmc, lib, tallies=None, zeff=None, duration=None, **names):
+    """'!Parameters.txt' as Save_output writes it (SO:489-506): the
+    parameter lines, then the ion's equilibrium charge (zeff, when given),
+    the energy loss Out_tot_E at the last node per layer thickness (from
+    the run's unnormalised tallies and nmc) and the run's duration [s]."""
+    import os
+    os.makedirs(folder, exist_ok=True)
+    L = parameter_lines(mat, ion, opt, nmc, lib, **names)
+    if zeff is not None:
+        L.append("Ion equilibrium charge:         " + f"{zeff:6.3f}".strip() + " [electron charge]")
+    if tallies is not None:
+        Se = float(tallies["Out_tot_E"][-1]) / float(nmc) / mat.layer
+        L.append("MC calculated energy loss (Se): " + f"{Se:9.2f}".strip() + " [eV/A]")
+    if duration is not None:
+        L.append("Duration of calculation:        " + duration_text(float(duration)))
+    L.append(_DASHES)
+    with open(os.path.join(folder, "!Parameters.txt"), "w") as f:
+        f.write("\n".join(L) + "\n")
+    return "!Parameters.txt"
