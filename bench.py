@@ -4,9 +4,9 @@ Workload: the Al2O3-like target (Egap 8.8 eV, 4 VB + Al L + O K oscillator
 sets, 3.95 g/cm^3, 100 A layer) hit by Au (Z 79) at 11 MeV/u, tallies on 1000
 time nodes (0-1000 fs, dt 1 fs) x 100 log radial bins (1-1000 A).  Tables are
 built once before timing (resident in HBM).  One step = trk_run_mc over
---iters-per-step Monte Carlo iterations (whole ion histories with all their
-cascades), with distinct iteration numbers so every step draws fresh random
-streams.  The metric is collision events per second (every SHI ionisation,
+--iters-per-step Monte Carlo iterations (default 10^4, the configuration's
+own count: whole ion histories with all their cascades), with distinct
+iteration numbers so every step draws fresh random streams.  The metric is collision events per second (every SHI ionisation,
 electron/hole elastic or inelastic collision, Auger or radiative decay and
 photoabsorption counts once).
 
@@ -343,7 +343,9 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--iters-per-step", type=int, default=4096)
+    ap.add_argument("--iters-per-step", type=int, default=None,
+                    help="Monte Carlo iterations per step (default: config2 10^4, the configuration's own count; "
+                         "config5 256)")
     ap.add_argument("--precision", choices=["mixed", "fp64"], default="mixed",
                     help="path whose events/s is the value (mixed: also runs the FP64 path for the comparison)")
     ap.add_argument("--no-compare", action="store_true",
@@ -366,6 +368,8 @@ def main():
                     help="only run a plain device-to-device copy of MIB MiB five times (the reference kernel "
                          "that scales HBM byte counters) and exit")
     args = ap.parse_args()
+    if args.iters_per_step is None:
+        args.iters_per_step = 256 if args.config == "config5" else 10000
     if args.calibrate_copy:
         import torch
         n = args.calibrate_copy * (1 << 20) // 4
