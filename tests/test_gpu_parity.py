@@ -269,6 +269,9 @@ def test_dsf_error_paths():
     bad = dsf.DsfTable(tables[0].E[::-1].copy(), tables[0].dE, tables[0].dL, tables[0].dL_emit, tables[0].dL_absorb)
     with pytest.raises(tg.TrekisError, match="increasing"):
         ctx.set_dsf(0, bad)
+    neg = dsf.DsfTable(tables[0].E, tables[0].dE, -tables[0].dL, tables[0].dL_emit, tables[0].dL_absorb)
+    with pytest.raises(tg.TrekisError, match="positive"):
+        ctx.set_dsf(0, neg)
     ctx.set_dsf(0, tables[0])
     ctx.set_dsf(1, tables[1])
     ctx.build_tables()

@@ -335,9 +335,13 @@ int trk_dsf_set(trk_ctx *c, int32_t particle, int32_t nE, int32_t nW, const doub
             return set_error(c, TRK_ERR_ARG, "DSF energies must be positive, finite and increasing (energy %d)", i);
     }
     const size_t cells = (size_t)nE * nW;
-    for (size_t k = 0; k < cells; ++k)
+    for (size_t k = 0; k < cells; ++k) {
         if (!std::isfinite(dE[k]) || dL[k] != dL[k] || dL_emit[k] != dL_emit[k] || dL_absorb[k] != dL_absorb[k])
             return set_error(c, TRK_ERR_ARG, "DSF rows hold a NaN or an infinite transfer (entry %zu)", k);
+        // cumulative mean free paths: positive (1e30 where nothing scatters)
+        if (!(dL[k] > 0.0) || !(dL_emit[k] > 0.0) || !(dL_absorb[k] > 0.0))
+            return set_error(c, TRK_ERR_ARG, "DSF mean free paths must be positive (entry %zu)", k);
+    }
     h.dsf_nW[particle] = nW;
     h.dsfE[particle].assign(E, E + nE);
     h.dsfdE[particle].assign(dE, dE + cells);
