@@ -81,6 +81,8 @@ def test_file_round_trip():
         assert np.array_equal(x, y)
     with pytest.raises(ValueError):
         dsf.read_dsf("30\n1.0 0.1 1.0\n")
+    with pytest.raises(ValueError, match="decreasing"):
+        dsf.process_dsf(E, dE[:, ::-1], s)
 
 
 W0 = (0.06, 0.10, 0.13)

@@ -86,6 +86,8 @@ def process_dsf(E, dE_raw, dsf_raw):
     dE_raw = np.asarray(dE_raw, dtype=np.float64)
     dsf_raw = np.where(np.asarray(dsf_raw, dtype=np.float64) < 1.0e-10, 0.0, dsf_raw)
     nE, nW = dE_raw.shape
+    if not np.all(np.diff(dE_raw, axis=1) < 0.0) or not np.all(np.diff(E) > 0.0):
+        raise ValueError("DSF blocks must list decreasing transferred energies, at increasing particle energies")
     out_dE = np.zeros((nE, nW))
     out_L = np.zeros((nE, nW))
     out_Le = np.zeros((nE, nW))
