@@ -514,8 +514,8 @@ def main():
                    "vs_fp64": compare},
         # achieved: the algorithmic bytes the kernel moves per launch (state
         # read and written per event, spawned particles, table gathers) over
-        # its mean launch time.  The survey's model also counts a per-time-node
-        # snapshot of every live particle, which the difference-array tallies
+        # its mean launch time.  A model of the reference's loop also counts a
+        # per-time-node snapshot of every live particle, which the difference-array tallies
         # never make; that figure is reported beside it, not as the fraction.
         "roofline": {"bound": "hbm", "achieved": achieved_moved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved_moved / HBM_PEAK_GBS, "traffic": traffic, "traffic_source": traffic_src,
