@@ -190,3 +190,29 @@ def test_batch_refuses_the_general_transfers(name):
     R, V = tg.radial_grid(run["NR"], run["Rmin"], run["Rmax"], 100.0)
     with pytest.raises(tg.TrekisError, match="batches do not"):
         Batch([c], R, V, capacity=1 << 16)
+
+
+@pytest.mark.parametrize("name", ["opt_dsf", "opt_beb", "opt_delta_cdf"])
+def test_ion_context_on_general_material_runs_like_a_full_context(name):
+    """An ion context on a material with DSF tables, BEB shells or the
+    Delta-CDF model shares those tables (and the material's model data):
+    trk_run_mc on it gives, bit for bit, the tallies of a full context of
+    the same ion."""
+    lib = tg.load_library()
+    mat, ion, opt, run = tg.config_material(name)
+    mat.prepare(lib)
+    R, V = tg.radial_grid(run["NR"], run["Rmin"], run["Rmax"], mat.layer)
+    full = tg.Context(mat, ion, opt)
+    full.build_tables()
+    other = abi.Ion.from_buffer_copy(ion)
+    other.E = ion.E * 0.5
+    base = tg.Context(mat, other, opt)
+    base.build_tables()
+    c = IonContext(base, ion)
+    c.build_tables()
+    say(f"[ion ctx] {name}: full context and ion context, {run['nmc']} iterations")
+    a = full.run_mc(run["nmc"], run["seed"], R, V)
+    b = c.run_mc(run["nmc"], run["seed"], R, V)
+    assert (a.n_events == b.n_events).all() and a.n_events.sum() > 0
+    for k in a.arrays:
+        assert np.array_equal(a[k], b[k]), k
