@@ -192,7 +192,7 @@ def test_batch_refuses_the_general_transfers(name):
         Batch([c], R, V, capacity=1 << 16)
 
 
-@pytest.mark.parametrize("name", ["opt_dsf", "opt_beb", "opt_delta_cdf", "opt_emission"])
+@pytest.mark.parametrize("name", ["opt_dsf", "opt_beb", "opt_delta_cdf", "opt_emission", "opt_combo"])
 def test_ion_context_on_general_material_runs_like_a_full_context(name):
     """An ion context on a material with DSF tables, BEB shells or the
     Delta-CDF model shares those tables (and the material's model data):

@@ -66,7 +66,7 @@ def _descriptor(m, ion, opt, nmc, seed, Nt, R, V, ms, dsf=None):
 
 @pytest.mark.skipif(not os.path.exists(DRIVER), reason="the Fortran bridge driver is built where the reference's "
                                                        "Objects.f90 is (make -C trekis-3_amd/fortran)")
-@pytest.mark.parametrize("name", ["test", "test_metal", "opt_emission", "opt_beb", "opt_dsf"])
+@pytest.mark.parametrize("name", ["test", "test_metal", "opt_emission", "opt_beb", "opt_dsf", "opt_combo"])
 def test_fortran_do_monte_carlo_equals_c_abi(name, tmp_path):
     lib = tg.load_library()
     mat, ion, opt, run = tg.config_material(name)

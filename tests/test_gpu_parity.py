@@ -24,15 +24,15 @@ FULL = ["config1", "config2"]       # the Al-like/Xe and Al2O3-like/Au workloads
 # the option branches the workloads above leave at their defaults: dispersion
 # kinds 2 and 3, ion charge models 1-4, a 300 K target, the plasmon integration
 # limit, logarithmic time grids, a Brandt-Kitagawa ion, emission through the
-# surface, the Delta-CDF model, BEB shells and DSF elastic scattering
-# (configs.OPTION_WORKLOADS)
+# surface, the Delta-CDF model, BEB shells, DSF elastic scattering and all of
+# them together (configs.OPTION_WORKLOADS)
 OPTIONS = ["opt_plasmon_pole", "opt_ritchie_warm", "opt_sg_warm", "opt_fixed_zeff", "opt_bk_ion", "opt_emission",
-           "opt_delta_cdf", "opt_beb", "opt_dsf"]
+           "opt_delta_cdf", "opt_beb", "opt_dsf", "opt_combo"]
 # table parity: test_sio2 has config5's tables (SiO2-like, U 20 MeV/u, phonon CDF, photons).
 # opt_ritchie_warm's full oracle tables take minutes (a deterministic pow per
 # q step); its rows are compared at sampled energies below instead.
 TABLES = ["test", "test_metal", "test_cdf", "test_polymer"] + FULL + ["config5"] + \
-    ["opt_plasmon_pole", "opt_sg_warm", "opt_fixed_zeff", "opt_bk_ion", "opt_delta_cdf", "opt_beb", "opt_dsf"]
+    ["opt_plasmon_pole", "opt_sg_warm", "opt_fixed_zeff", "opt_bk_ion", "opt_delta_cdf", "opt_beb", "opt_dsf", "opt_combo"]
 
 
 @pytest.mark.parametrize("name", TABLES)

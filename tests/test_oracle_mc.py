@@ -22,7 +22,7 @@ import pytest
 import trekis_gpu as tg
 from oracle import Oracle, dp
 
-CONFIGS = ["test", "test_metal", "opt_emission", "opt_delta_cdf", "opt_beb", "opt_dsf"]
+CONFIGS = ["test", "test_metal", "opt_emission", "opt_delta_cdf", "opt_beb", "opt_dsf", "opt_combo"]
 
 
 @pytest.fixture(scope="module", params=CONFIGS)
@@ -135,8 +135,8 @@ def test_emission_rules(runs):
     Out_E_Em at each node count and sum E - W over the emissions before it,
     and every iteration with an emission adds exactly one to the integral
     of Out_Ee_vs_E_Em over the energy grid Out_R/10."""
-    if runs["name"] != "opt_emission":
-        pytest.skip("emission is on only in opt_emission")
+    if not runs["mat"].work_function > 0.0:
+        pytest.skip("emission is on only in opt_emission and opt_combo")
     a, mat, R, tim = runs["a"], runs["mat"], runs["R"], runs["tim"]
     W, H = mat.work_function, mat.bar_height
     E1, g = ctypes.c_double(), ctypes.c_double()

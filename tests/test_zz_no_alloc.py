@@ -15,7 +15,7 @@ from gpu_helpers import say
 pytestmark = pytest.mark.gpu
 
 
-@pytest.mark.parametrize("name", ["test", "test_metal", "opt_emission", "opt_delta_cdf", "opt_beb", "opt_dsf"])
+@pytest.mark.parametrize("name", ["test", "test_metal", "opt_emission", "opt_delta_cdf", "opt_beb", "opt_dsf", "opt_combo"])
 def test_noalloc_equals_allocating_entry_point(name):
     lib = tg.load_library()
     mat, ion, opt, run = tg.config_material(name)

@@ -468,6 +468,12 @@ def config_material(name):
 #                     ion keeps their CDF
 #   opt_delta_cdf     the Delta-CDF model of the electron and hole inelastic
 #                     cross sections (kind_of_DR = 4, CS:952-956, 1894-1896)
+#   opt_dsf           DSF elastic scattering (kind_of_EMFP = 2, a 12th entry
+#                     True) on the synthetic DSF tables of dsf.synthetic_dsf,
+#                     300 K
+#   opt_combo         the branches together: Delta-CDF, a BEB shell, DSF
+#                     elastic scattering, emission, a Brandt-Kitagawa ion with
+#                     Bohr charge, 300 K, logarithmic time grid of ratio 2
 #   opt_emission      emission through the surface Z = 0 (a 10th entry: work
 #                     function [eV], barrier length [A], barrier height [eV];
 #                     calculate_emission, MC:2477-2513): electrons above 15 eV
@@ -483,6 +489,7 @@ OPTION_WORKLOADS = {
     "opt_delta_cdf": ("test", 4, 0, 8.0, 0.0, 0, 0, 1.0),
     "opt_beb": ("test", 1, 0, 8.0, 0.0, 0, 0, 1.0, 0, None, {0: 105.0, 2: 700.0}),
     "opt_dsf": ("test", 1, 0, 8.0, 300.0, 0, 0, 1.0, 0, None, {}, True),
+    "opt_combo": ("test", 4, 1, 8.0, 300.0, 0, 1, 2.0, 1, (4.5, 10.0, 10.0), {0: 105.0}, True),
 }
 
 
