@@ -43,7 +43,8 @@ pytestmark = pytest.mark.gpu
 RADIAL = {"Out_ne": "Out_ne", "Out_Ee": "Out_ne", "Out_Elat": "Out_ne",
           "Out_nh": "Out_nh", "Out_Eh": "Out_nh", "Out_Ehkin": "Out_nh"}
 MIN_COUNTS = 1000
-CASES = {"test_metal": 256, "config2": 512, "config5": 16, "test_cdf": 256, "opt_delta_cdf": 64, "opt_dsf": 128}
+CASES = {"test_metal": 256, "config2": 512, "config5": 16, "test_cdf": 256, "opt_delta_cdf": 64, "opt_dsf": 128,
+         "opt_combo": 128}
 
 
 def _mixed_context(name):
